@@ -1,0 +1,135 @@
+/*
+ * C ABI of the MI355X KuQCD library (libkuqcd.so): plain pointers and sizes,
+ * callable from C, ctypes, cgo, JNI.
+ *
+ * Part 1 re-exports the reference API (include/kuqcd.h:29-34) with C linkage.
+ * Part 2 adds the MI355X extensions: precision modes, per-shift FP64
+ * verification and refinement, multi-right-hand-side batching, operator-level
+ * entry points, synthetic inputs and timing hooks. Host buffers always use the
+ * reference's host layouts (kuqcd.cpp:103-169):
+ *   gauge  : h[2*(l + 9*(mu + 4*i)) + ri], i = parity*Vh + checkerboard index
+ *   vector : h[2*(c + 3*s) + ri], s = checkerboard index of one parity
+ */
+#ifndef KUQCD_HIP_H
+#define KUQCD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "kuqcd_enum.h"
+
+#define KQ_API __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------ Part 1: reference API, C linkage */
+/* replaces kuqcd::kuqcd_init (kuqcd.h:29, kuqcd.cpp:288-301): grid from stdin */
+KQ_API KuQCDStatus kq_init(void);
+/* kuqcd_init with the grid given directly instead of read from stdin */
+KQ_API KuQCDStatus kq_init_grid(const int grid[4]);
+/* replaces kuqcd::kuqcd_load_gaugefield (kuqcd.h:30, kuqcd.cpp:303-402) */
+KQ_API KuQCDStatus kq_load_gaugefield(void *h_gauge, KuQCDGaugeParam *param);
+/* replaces kuqcd::kuqcd_solve_multishift (kuqcd.h:31, kuqcd.cpp:404-445) */
+KQ_API KuQCDStatus kq_solve_multishift(void **x, void *b, KuQCDInvertParam *param);
+/* replaces kuqcd::kuqcd_gauge_free (kuqcd.h:33, kuqcd.cpp:447-476) */
+KQ_API void kq_gauge_free(void);
+/* replaces kuqcd::kuqcd_end (kuqcd.h:34, kuqcd.cpp:477-487) */
+KQ_API void kq_end(void);
+
+/* ------------------------------------------------ Part 2: extensions (no reference counterpart) */
+typedef enum {
+    KQ_MODE_REFERENCE = 0,  /* the reference's method in the loaded gauge precision */
+    KQ_MODE_FP64 = 1,       /* plain FP64 multishift CG (double gauge) */
+    KQ_MODE_MIXED_FP32 = 2, /* FP32 links/vectors, FP64 reliable updates + per-shift refinement */
+    KQ_MODE_MIXED_FP16 = 3  /* FP16 links (+ per-link FP32 scale, 12-real long links), FP32 vectors */
+} KqMode;
+
+typedef struct {
+    int mode;              /* KqMode */
+    double reliable_delta; /* reliable update when |r| drops below delta * |r| at the last one (0.1) */
+    int refine;            /* 1: FP64 true-residual check of every shift + batched refinement */
+    double rel_tol;        /* per-shift target of ||b - A_k x_k|| / ||b||; 0: sqrt(tol)/||b|| */
+    int learned;           /* 1: learned-preconditioner flexible CG for the lightest shift */
+    int chunk;             /* iterations replayed between host checks (0: default) */
+    int nr;                /* right-hand sides per thread of the batched hopping term (0: autotune) */
+    int nt;                /* threads per block of the hopping term (0: autotune) */
+    int verbose;           /* 0 silent, 1 iteration summary (reference print), 2 details */
+} KqSolveOptions;
+
+#define KQ_MAX_SHIFT 32
+typedef struct {
+    int iter;                          /* multishift iterations (reference's param->iter) */
+    int reliable_updates;
+    int refine_rounds;
+    int refine_iter[KQ_MAX_SHIFT];     /* refinement iterations per shift */
+    double true_rel_res[KQ_MAX_SHIFT]; /* FP64 ||b - A_k x_k|| / ||b|| per shift */
+    double seconds;                    /* wall time of the whole call, host buffers in and out */
+    double seconds_multishift;
+    double seconds_refine;
+    int learned_used;                  /* 1 if the learned refinement was accepted */
+    int learned_fallback;              /* 1 if it was rejected and plain CG refinement used */
+} KqSolveReport;
+
+KQ_API void kq_default_options(KqSolveOptions *opt);
+
+/* Multishift solve with options; same inputs and shift/tol conventions as
+ * kq_solve_multishift. rep may be NULL. */
+KQ_API KuQCDStatus kq_solve_multishift_ex(void **x, void *b, KuQCDInvertParam *param, const KqSolveOptions *opt,
+                                   KqSolveReport *rep);
+
+/* nrhs independent sources solved together (one gauge field): b[r] host
+ * vectors, x[r*num_shift + k] host solutions, param->shifts holds
+ * nrhs*num_shift shifts (reference convention per source). rep: nrhs reports (may be NULL). */
+KQ_API KuQCDStatus kq_solve_multishift_batched(int nrhs, void **x, void **b, KuQCDInvertParam *param,
+                                        const KqSolveOptions *opt, KqSolveReport *rep);
+
+/* dst = D src (hopping term, dirac_hisq.cpp:409-455) with the loaded gauge:
+ * src on parity 1-dst_parity, host buffers in the gauge precision;
+ * mode picks the device link/vector format (KqMode; REFERENCE = as loaded). */
+KQ_API KuQCDStatus kq_dslash(const void *src, void *dst, int dst_parity, int mode);
+/* nrhs vectors back to back (host layout each); nr/nt as in KqSolveOptions */
+KQ_API KuQCDStatus kq_dslash_batched(int nrhs, const void *src, void *dst, int dst_parity, int mode, int nr, int nt);
+/* dst = sigma*src - D D src on parity `parity` (the CGM operator A) */
+KQ_API KuQCDStatus kq_mdaggm(const void *src, void *dst, int parity, double sigma, int mode);
+
+/* ---- timing hooks (device-resident inputs, HIP events on the library stream) */
+typedef struct {
+    double ms;             /* average time per application */
+    double bytes;          /* algorithmic bytes per application */
+    double gbs;            /* bytes / ms */
+    double gflops;         /* 1146 flop per site per right-hand side */
+    double hbm_frac;       /* gbs / 8000 */
+    int nr, nt;            /* launch configuration used */
+} KqBenchResult;
+KQ_API KuQCDStatus kq_bench_dslash(int mode, int nrhs, int nr, int nt, int iters, KqBenchResult *out);
+/* plain device copy of `bytes` bytes (HBM reference point) */
+KQ_API KuQCDStatus kq_bench_copy(size_t bytes, int iters, KqBenchResult *out);
+/* pick nr, nt for (mode, nrhs) on the loaded lattice by timing; cached */
+KQ_API KuQCDStatus kq_autotune_dslash(int mode, int nrhs, int *nr, int *nt);
+
+/* ---- synthetic inputs (reference host layouts, double precision) */
+/* kind 0: Haar-random SU(3); kind 1: exp(i eps H), H traceless Hermitian Gaussian.
+ * fat = fat_scale * eta * G(seed_fat); long = c_naik * eta * W(x)W(x+mu)W(x+2mu), W = G(seed_long);
+ * either output may be NULL. phases = 1 folds the staggered phases in. */
+KQ_API int kq_gen_gauge(double *fat, double *lng, const unsigned lat[4], int kind, double eps, double fat_scale,
+                 double c_naik, uint64_t seed_fat, uint64_t seed_long, int phases);
+/* complex Gaussian source, E|z|^2 = 1, on the Vh sites of `parity` */
+KQ_API int kq_gen_source(double *b, const unsigned lat[4], int parity, uint64_t seed);
+
+/* ---- misc */
+KQ_API const char *kq_last_error(void);
+KQ_API void kq_set_verbosity(int v);
+KQ_API int kq_device_count(void);
+/* host check of the fast division used by the kernels' index math: returns
+ * the number of n in [0, n_max) with n / d != the fast quotient */
+KQ_API long kq_selftest_fastdiv(uint32_t d, uint32_t n_max);
+/* checkerboard index of the neighbour of site s (parity q) at hop h along mu,
+ * computed by the kernels' index math on the host */
+KQ_API long kq_neighbour_index(const int lat[4], int q, long s, int mu, int h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KUQCD_HIP_H */

@@ -1,0 +1,469 @@
+// Kernels of one multishift CG iteration (CGM::Invert, inverter_cgm.cpp:42-78)
+// with the recurrence scalars kept in device memory.
+//
+// Per iteration:  begin -> D (p0 -> tmp) -> D + axmy + <p0,s> -> scalars1 ->
+//                 r += b0 s, |r|^2 -> scalars2 -> x/p update of every live shift
+// Reductions are two-stage and deterministic: each workgroup writes one FP64
+// partial sum, and one workgroup adds the partials in a fixed order.
+#include "kq_cgm_kernels.h"
+
+namespace kq {
+
+// ------------------------------------------------------------ reduce helpers
+// Sum part[0..n) in a fixed order with one workgroup of 256 threads.
+__device__ inline double block_reduce_fixed(const double *part, uint32_t n)
+{
+    __shared__ double lds[256];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        acc += part[i];
+    lds[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    return lds[0];
+}
+
+// ------------------------------------------------------------ begin
+template <typename S> __global__ void k_cgm_begin(CgmState<S> *st, int nrhs, int *rlist, int *nact, int *any)
+{
+    if (threadIdx.x != 0)
+        return;
+    int n = 0;
+    for (int r = 0; r < nrhs; ++r) {
+        const int live = st[r].stop == 0;
+        st[r].run = live;
+        if (live)
+            rlist[n++] = r;
+    }
+    *nact = n;
+    *any = n > 0;
+}
+
+template <typename S> void cgm_begin(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_cgm_begin<S>, dim3(1), dim3(64), 0, stream, st, L.nrhs, L.rlist, L.nact, L.any);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ scalars 1
+// pAp = <p0, s>;  beta_0 = -|r|^2 / pAp;  zeta/beta recurrences of the shifted
+// systems (inverter_cgm.cpp:47-57). The FP contraction is switched off so the
+// recurrence rounds exactly as written; the double literal 1.0 of the
+// reference keeps its type (the denominator is formed in double for S=float).
+template <typename S>
+__global__ void __launch_bounds__(256) k_cgm_scalars1(CgmState<S> *st, const int *rlist, const int *nact,
+                                                      const double *part, uint32_t nblk)
+{
+#pragma clang fp contract(off)
+    const int slot = blockIdx.x;
+    if (*nact <= slot)
+        return;
+    const int r = rlist[slot];
+    const double pAp_d = block_reduce_fixed(part + (size_t)r * nblk, nblk);
+    if (threadIdx.x != 0)
+        return;
+    CgmState<S> &c = st[r];
+    const S pAp = (S)pAp_d;
+    const S B0 = -c.rr / pAp;
+    c.B[0] = B0;
+    for (int j = 1; j < c.maxTerm; ++j) {
+        const S num = c.Bm1 * c.Zm1[j];
+        const S t1 = B0 * c.a[0] * (c.Zm1[j] - c.Z[j]);
+        const S t2 = c.Zm1[j] * c.Bm1;
+        const S t3 = c.shift[j] * B0;
+        const S rr1 = (S)((double)num / ((double)t1 + (double)t2 * (1.0 - (double)t3)));
+        c.Zm1[j] = c.Z[j];
+        c.Z[j] = c.Z[j] * rr1;
+        c.B[j] = B0 * rr1;
+    }
+    c.Bm1 = B0;
+}
+
+template <typename S> void cgm_scalars1(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_cgm_scalars1<S>, dim3(L.nrhs), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_a,
+                       L.nblk_dsl);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ scalars 2
+// |r|^2 new, alpha recurrences, drop converged shifts from the end,
+// iteration count and stop flag (inverter_cgm.cpp:58-78).
+template <typename S>
+__global__ void __launch_bounds__(256) k_cgm_scalars2(CgmState<S> *st, const int *rlist, const int *nact,
+                                                      const double *part, uint32_t nblk)
+{
+#pragma clang fp contract(off)
+    const int slot = blockIdx.x;
+    if (*nact <= slot)
+        return;
+    const int r = rlist[slot];
+    const double l2_d = block_reduce_fixed(part + (size_t)r * nblk, nblk);
+    if (threadIdx.x != 0)
+        return;
+    CgmState<S> &c = st[r];
+    const S lambda2 = (S)l2_d;
+    c.a[0] = lambda2 / c.rr;
+    c.rr = lambda2;
+    for (int j = 1; j < c.maxTerm; ++j)
+        c.a[j] = c.a[0] * c.Z[j] * c.B[j] / (c.Zm1[j] * c.B[0]);
+    c.maxTermUpd = c.maxTerm;
+    S lambda;
+    do {
+        lambda = c.Z[c.maxTerm - 1] * c.Z[c.maxTerm - 1] * lambda2;
+        if ((double)lambda < c.tol)
+            c.maxTerm--;
+    } while (c.maxTerm > 0 && (double)lambda < c.tol);
+    c.iter += 1;
+    if (c.maxTerm <= 0 || c.iter >= c.maxIter)
+        c.stop = 1;
+}
+
+template <typename S> void cgm_scalars2(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_cgm_scalars2<S>, dim3(L.nrhs), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_b,
+                       L.nblk);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ r += b0 s, |r|^2
+template <typename S, typename VR>
+__global__ void __launch_bounds__(kBlasNT) k_axpy_norm(const CgmState<S> *st, Vec<VR> r, Vec<VR> s,
+                                                       const int *rlist, const int *nact, double *part,
+                                                       uint32_t nblk)
+{
+    using V2 = c2_t<VR>;
+    __shared__ double lds[kBlasNT / kWave];
+    const int slot = blockIdx.y;
+    if (*nact <= slot)
+        return;
+    const int rh = rlist[slot];
+    const VR b0 = (VR)st[rh].B[0];
+    const uint32_t i = blockIdx.x * kBlasNT + threadIdx.x;
+    double acc = 0.0;
+    if (i < r.Vh) {
+        V2 rv[3], sv[3];
+        r.load(rh, i, rv);
+        s.load(rh, i, sv);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            rv[c].x += b0 * sv[c].x;
+            rv[c].y += b0 * sv[c].y;
+            acc += (double)rv[c].x * (double)rv[c].x + (double)rv[c].y * (double)rv[c].y;
+        }
+        r.store(rh, i, rv);
+    }
+    const double tot = block_sum<kBlasNT>(acc, lds);
+    if (threadIdx.x == 0)
+        part[(size_t)rh * nblk + blockIdx.x] = tot;
+}
+
+template <typename S, typename VR>
+void cgm_axpy_norm(CgmState<S> *st, Vec<VR> r, Vec<VR> s, const CgmLaunch &L, hipStream_t stream)
+{
+    hipLaunchKernelGGL((k_axpy_norm<S, VR>), dim3(L.nblk, L.nrhs), dim3(kBlasNT), 0, stream, st, r, s, L.rlist,
+                       L.nact, L.part_b, L.nblk);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ x/p update
+template <typename S, typename VR, typename XR>
+__global__ void __launch_bounds__(kBlasNT) k_xp_update(const CgmState<S> *st, XR *x, VR *p, Vec<VR> r, int nshift,
+                                                       const int *rlist, const int *nact)
+{
+    using V2 = c2_t<VR>;
+    using X2 = c2_t<XR>;
+    const int slot = blockIdx.y;
+    if (*nact <= slot)
+        return;
+    const int rh = rlist[slot];
+    const uint32_t i = blockIdx.x * kBlasNT + threadIdx.x;
+    const uint32_t Vh = r.Vh;
+    if (i >= Vh)
+        return;
+    const CgmState<S> &c = st[rh];
+    const int m = c.maxTermUpd;
+    V2 rv[3];
+    r.load(rh, i, rv);
+    for (int k = 0; k < m; ++k) {
+        const size_t base = ((size_t)rh * nshift + k) * 3 * Vh + i;
+        const XR bk = (XR)c.B[k];
+        const VR zk = (VR)c.Z[k], ak = (VR)c.a[k];
+        V2 *pp = reinterpret_cast<V2 *>(p) + base;
+        X2 *xx = reinterpret_cast<X2 *>(x) + base;
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            const V2 pv = pp[(size_t)cc * Vh];
+            X2 xv = xx[(size_t)cc * Vh];
+            xv.x -= bk * (XR)pv.x;
+            xv.y -= bk * (XR)pv.y;
+            xx[(size_t)cc * Vh] = xv;
+            V2 np;
+            np.x = zk * rv[cc].x + ak * pv.x;
+            np.y = zk * rv[cc].y + ak * pv.y;
+            pp[(size_t)cc * Vh] = np;
+        }
+    }
+}
+
+template <typename S, typename VR, typename XR>
+void cgm_xp_update(CgmState<S> *st, XR *x, VR *p, Vec<VR> r, int nshift, const CgmLaunch &L, hipStream_t stream)
+{
+    hipLaunchKernelGGL((k_xp_update<S, VR, XR>), dim3(L.nblk, L.nrhs), dim3(kBlasNT), 0, stream, st,
+                       reinterpret_cast<XR *>(x), reinterpret_cast<VR *>(p), r, nshift, L.rlist, L.nact);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ init
+template <typename S> __global__ void __launch_bounds__(256) k_cgm_init(CgmState<S> *st, const double *part, uint32_t nblk)
+{
+    const int r = blockIdx.x;
+    const double rr = block_reduce_fixed(part + (size_t)r * nblk, nblk);
+    if (threadIdx.x != 0)
+        return;
+    CgmState<S> &c = st[r];
+    c.rr = (S)rr;
+    c.Bm1 = (S)1.0;
+    for (int k = 0; k < kMaxShift; ++k) {
+        c.a[k] = (S)0.0;
+        c.B[k] = (S)1.0;
+        c.Z[k] = (S)1.0;
+        c.Zm1[k] = (S)1.0;
+    }
+    c.maxTerm = c.nshift;
+    c.maxTermUpd = c.nshift;
+    c.maxTermFold = c.nshift;
+    c.r2max = rr;
+    c.iter = 0;
+    // the reference's do-while runs at least one iteration even for max_iter <= 0
+    c.stop = (c.nshift <= 0) ? 1 : 0;
+    c.run = 0;
+}
+
+template <typename S> void cgm_init_state(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_cgm_init<S>, dim3(L.nrhs), dim3(256), 0, stream, st, L.part_b, L.nblk);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ reliable updates
+template <typename VR>
+__global__ void __launch_bounds__(kBlasNT) k_rel_residual(const double *b, const double *ax, Vec<VR> r_lo,
+                                                          const int *rel_list, const int *rel_n, double *part,
+                                                          uint32_t nblk)
+{
+    using V2 = c2_t<VR>;
+    __shared__ double lds[kBlasNT / kWave];
+    const int slot = blockIdx.y;
+    if (*rel_n <= slot)
+        return;
+    const int rh = rel_list[slot];
+    const uint32_t Vh = r_lo.Vh;
+    const uint32_t i = blockIdx.x * kBlasNT + threadIdx.x;
+    double acc = 0.0;
+    if (i < Vh) {
+        V2 o[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const size_t e = ((size_t)rh * 3 + c) * Vh + i;
+            const double2 bb = reinterpret_cast<const double2 *>(b)[e];
+            const double2 aa = reinterpret_cast<const double2 *>(ax)[e];
+            const double rx = bb.x - aa.x, ry = bb.y - aa.y;
+            acc += rx * rx + ry * ry;
+            o[c].x = (VR)rx;
+            o[c].y = (VR)ry;
+        }
+        r_lo.store(rh, i, o);
+    }
+    const double tot = block_sum<kBlasNT>(acc, lds);
+    if (threadIdx.x == 0)
+        part[(size_t)rh * nblk + blockIdx.x] = tot;
+}
+
+template <typename VR>
+void rel_residual(const double *b, const double *ax, Vec<VR> r_lo, const int *rel_list, const int *rel_n, int nsys,
+                  double *part, uint32_t nblk, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_rel_residual<VR>, dim3(nblk, nsys), dim3(kBlasNT), 0, stream, b, ax, r_lo, rel_list, rel_n,
+                       part, nblk);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_rel_set_rr(CgmState<S> *st, const int *rel_list, const int *rel_n,
+                                                    const double *part, uint32_t nblk)
+{
+    const int slot = blockIdx.x;
+    if (*rel_n <= slot)
+        return;
+    const int rh = rel_list[slot];
+    const double rr = block_reduce_fixed(part + (size_t)rh * nblk, nblk);
+    if (threadIdx.x == 0) {
+        st[rh].rr = (S)rr;
+        st[rh].r2max = rr;
+    }
+}
+
+template <typename S>
+void rel_set_rr(CgmState<S> *st, const int *rel_list, const int *rel_n, int nsys, const double *part, uint32_t nblk,
+                hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_rel_set_rr<S>, dim3(nsys), dim3(256), 0, stream, st, rel_list, rel_n, part, nblk);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+template <typename S, typename VR>
+__global__ void __launch_bounds__(kBlasNT) k_rel_fold(const CgmState<S> *st, double *x_hi, VR *x_lo, int nshift,
+                                                      uint32_t Vh, const int *rel_list, const int *rel_n)
+{
+    using V2 = c2_t<VR>;
+    const int slot = blockIdx.y;
+    if (*rel_n <= slot)
+        return;
+    const int rh = rel_list[slot];
+    const size_t n3 = (size_t)3 * Vh;
+    const size_t i = (size_t)blockIdx.x * kBlasNT + threadIdx.x;
+    if (i >= n3)
+        return;
+    const int m = st[rh].maxTermFold;
+    for (int k = 0; k < m; ++k) {
+        const size_t e = ((size_t)rh * nshift + k) * n3 + i;
+        V2 *xl = reinterpret_cast<V2 *>(x_lo) + e;
+        double2 *xh = reinterpret_cast<double2 *>(x_hi) + e;
+        const V2 l = *xl;
+        double2 h = *xh;
+        h.x += (double)l.x;
+        h.y += (double)l.y;
+        *xh = h;
+        V2 z;
+        z.x = 0;
+        z.y = 0;
+        *xl = z;
+    }
+}
+
+template <typename S, typename VR>
+void rel_fold(CgmState<S> *st, double *x_hi, VR *x_lo, int nshift, uint32_t Vh, const int *rel_list,
+              const int *rel_n, int nsys, hipStream_t stream)
+{
+    const size_t n3 = (size_t)3 * Vh;
+    hipLaunchKernelGGL((k_rel_fold<S, VR>), dim3((unsigned)((n3 + kBlasNT - 1) / kBlasNT), nsys), dim3(kBlasNT), 0,
+                       stream, st, x_hi, x_lo, nshift, Vh, rel_list, rel_n);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ true residuals
+__global__ void __launch_bounds__(kBlasNT) k_resid2(const double2 *b, const double2 *ax, uint32_t Vh, double *part,
+                                                    uint32_t nblk)
+{
+    __shared__ double lds[kBlasNT / kWave];
+    const int k = blockIdx.y;
+    const uint32_t i = blockIdx.x * kBlasNT + threadIdx.x;
+    double acc = 0.0;
+    if (i < Vh) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double2 bb = b[(size_t)c * Vh + i];
+            const double2 aa = ax[((size_t)k * 3 + c) * Vh + i];
+            const double rx = bb.x - aa.x, ry = bb.y - aa.y;
+            acc += rx * rx + ry * ry;
+        }
+    }
+    const double tot = block_sum<kBlasNT>(acc, lds);
+    if (threadIdx.x == 0)
+        part[(size_t)k * nblk + blockIdx.x] = tot;
+}
+
+void resid2_partial(const double *b, const double *ax, int nk, uint32_t Vh, double *part, uint32_t nblk,
+                    hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_resid2, dim3(nblk, nk), dim3(kBlasNT), 0, stream, reinterpret_cast<const double2 *>(b),
+                       reinterpret_cast<const double2 *>(ax), Vh, part, nblk);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ generic dot
+template <typename VR>
+__global__ void __launch_bounds__(kBlasNT) k_dot_partial(Vec<VR> x, Vec<VR> y, double *part, uint32_t nblk)
+{
+    using V2 = c2_t<VR>;
+    __shared__ double lds[kBlasNT / kWave];
+    const int rh = blockIdx.y;
+    const uint32_t i = blockIdx.x * kBlasNT + threadIdx.x;
+    double acc = 0.0;
+    if (i < x.Vh) {
+        V2 a[3], b[3];
+        x.load(rh, i, a);
+        y.load(rh, i, b);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            acc += (double)a[c].x * (double)b[c].x + (double)a[c].y * (double)b[c].y;
+    }
+    const double tot = block_sum<kBlasNT>(acc, lds);
+    if (threadIdx.x == 0)
+        part[(size_t)rh * nblk + blockIdx.x] = tot;
+}
+
+template <typename VR>
+void vec_dot_partial(Vec<VR> x, Vec<VR> y, int nrhs, double *part, uint32_t nblk, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_dot_partial<VR>, dim3(nblk, nrhs), dim3(kBlasNT), 0, stream, x, y, part, nblk);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(256) k_reduce_partials(const double *part, uint32_t nblk, double *out)
+{
+    const int r = blockIdx.x;
+    const double v = block_reduce_fixed(part + (size_t)r * nblk, nblk);
+    if (threadIdx.x == 0)
+        out[r] = v;
+}
+
+void reduce_partials(const double *part, uint32_t nblk, int nrhs, double *out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nrhs), dim3(256), 0, stream, part, nblk, out);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ instantiations
+#define KQ_INST_S(S)                                                                         \
+    template void cgm_begin<S>(CgmState<S> *, const CgmLaunch &, hipStream_t);               \
+    template void cgm_scalars1<S>(CgmState<S> *, const CgmLaunch &, hipStream_t);            \
+    template void cgm_scalars2<S>(CgmState<S> *, const CgmLaunch &, hipStream_t);            \
+    template void cgm_init_state<S>(CgmState<S> *, const CgmLaunch &, hipStream_t);
+KQ_INST_S(double)
+KQ_INST_S(float)
+template void cgm_axpy_norm<double, double>(CgmState<double> *, Vec<double>, Vec<double>, const CgmLaunch &,
+                                            hipStream_t);
+template void cgm_axpy_norm<float, float>(CgmState<float> *, Vec<float>, Vec<float>, const CgmLaunch &,
+                                          hipStream_t);
+template void cgm_axpy_norm<double, float>(CgmState<double> *, Vec<float>, Vec<float>, const CgmLaunch &,
+                                           hipStream_t);
+template void cgm_xp_update<double, double, double>(CgmState<double> *, double *, double *, Vec<double>, int,
+                                                    const CgmLaunch &, hipStream_t);
+template void cgm_xp_update<float, float, float>(CgmState<float> *, float *, float *, Vec<float>, int,
+                                                 const CgmLaunch &, hipStream_t);
+template void cgm_xp_update<double, float, float>(CgmState<double> *, float *, float *, Vec<float>, int,
+                                                  const CgmLaunch &, hipStream_t);
+template void rel_residual<float>(const double *, const double *, Vec<float>, const int *, const int *, int,
+                                   double *, uint32_t, hipStream_t);
+template void rel_residual<double>(const double *, const double *, Vec<double>, const int *, const int *, int,
+                                   double *, uint32_t, hipStream_t);
+template void rel_set_rr<double>(CgmState<double> *, const int *, const int *, int, const double *, uint32_t,
+                                 hipStream_t);
+template void rel_set_rr<float>(CgmState<float> *, const int *, const int *, int, const double *, uint32_t,
+                                hipStream_t);
+template void rel_fold<double, float>(CgmState<double> *, double *, float *, int, uint32_t, const int *, const int *,
+                                      int, hipStream_t);
+template void rel_fold<double, double>(CgmState<double> *, double *, double *, int, uint32_t, const int *,
+                                       const int *, int, hipStream_t);
+template void rel_fold<float, float>(CgmState<float> *, double *, float *, int, uint32_t, const int *, const int *,
+                                     int, hipStream_t);
+template void vec_dot_partial<double>(Vec<double>, Vec<double>, int, double *, uint32_t, hipStream_t);
+template void vec_dot_partial<float>(Vec<float>, Vec<float>, int, double *, uint32_t, hipStream_t);
+
+}  // namespace kq

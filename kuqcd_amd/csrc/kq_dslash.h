@@ -1,0 +1,41 @@
+// Host-side interface of the HISQ even-odd hopping-term kernels.
+#pragma once
+
+#include "kq_fields.h"
+
+namespace kq {
+
+enum DslashEpilogue {
+    EPI_STORE = 0,     // dst = D src
+    EPI_AXMY_DOT = 1,  // dst = sigma[r] * aux - D src ; partial[r][blk] = Re<aux, dst>
+};
+
+template <class LF, class LL, typename SR> struct DslashArgs {
+    using R = typename LF::R;
+    using V2 = c2_t<SR>;
+    Geom g;
+    int dst_parity;
+    LF fat;
+    LL lng;
+    Vec<SR> src, dst, aux;
+    const double *sigma;  // per right-hand side (EPI_AXMY_DOT)
+    double *partial;      // [nrhs_total][nblk] (EPI_AXMY_DOT)
+    const int *run;       // device flag; kernel does nothing when *run == 0 (may be null)
+    const int *rlist;     // compact list of active right-hand sides (may be null = identity)
+    const int *nact;      // number of entries of rlist in use (may be null = nrhs)
+    int nrhs;             // right-hand sides addressed by this launch
+    uint32_t nblk;        // grid.x, the partial-sum stride
+};
+
+// Launch D over all sites of parity dst_parity for every right-hand side.
+// nr = right-hand sides handled per thread (1, 2, 4 or 8), nt = threads per block.
+template <class LF, class LL, typename SR>
+void launch_dslash(DslashArgs<LF, LL, SR> a, int epi, int nr, int nt, hipStream_t st);
+
+inline uint32_t dslash_nblk(uint32_t Vh, int nt)
+{
+    uint32_t nb = (Vh + nt - 1) / nt;
+    return nb;
+}
+
+}  // namespace kq

@@ -1,0 +1,193 @@
+// HISQ even-odd hopping term (fat 1-hop + Naik 3-hop) on gfx950.
+//
+// One thread per output site (and per block of NR right-hand sides): the
+// 16 links of the site are each read once from HBM with coalesced
+// structure-of-arrays loads and applied to the NR gathered neighbour vectors,
+// so a batched launch reads every link once for all of its right-hand sides.
+// Periodic boundaries are resolved in-kernel from the site coordinates; no
+// neighbour tables are read. Follows DiracHISQ::Dslash/FwdDslash/BwdDslash
+// (dirac_hisq.cpp:409-502) with the 1/2 and c3000 factors folded into the
+// device links.
+#include "kq_dslash.h"
+
+namespace kq {
+
+template <class A, class LK, int NR, int MU, int H>
+__device__ __forceinline__ void hop_fwd(const A &a, const LK &lk, uint32_t s, const SiteCoord &sc, const int *rr,
+                                        typename A::R acc[NR][6])
+{
+    using R = typename A::R;
+    c2_t<R> U[9];
+    lk.load(a.dst_parity, MU, s, U);
+    const uint32_t n = nb_index<MU>(a.g, s, sc, H);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        typename A::V2 v[3];
+        a.src.load(rr[j], n, v);
+        umv_add(U, v, acc[j]);
+    }
+}
+
+template <class A, class LK, int NR, int MU, int H>
+__device__ __forceinline__ void hop_bwd(const A &a, const LK &lk, uint32_t s, const SiteCoord &sc, const int *rr,
+                                        typename A::R acc[NR][6])
+{
+    using R = typename A::R;
+    const uint32_t n = nb_index<MU>(a.g, s, sc, -H);
+    c2_t<R> U[9];
+    lk.load(1 - a.dst_parity, MU, n, U);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        typename A::V2 v[3];
+        a.src.load(rr[j], n, v);
+        udmv_sub(U, v, acc[j]);
+    }
+}
+
+template <class LF, class LL, typename SR, int NR, int EPI, int NT>
+__global__ void __launch_bounds__(NT) k_dslash(const DslashArgs<LF, LL, SR> a)
+{
+    using A = DslashArgs<LF, LL, SR>;
+    using R = typename A::R;
+    using V2 = c2_t<SR>;
+    __shared__ double lds[NT / kWave];
+    if (a.run != nullptr && *a.run == 0)
+        return;  // uniform over the grid
+    const int nact = a.nact ? *a.nact : a.nrhs;
+    const int j0 = blockIdx.y * NR;
+    if (j0 >= nact)
+        return;  // uniform over the block
+    int rr[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const int jj = (j0 + j < nact) ? j0 + j : nact - 1;  // duplicate the last one as padding
+        rr[j] = a.rlist ? a.rlist[jj] : jj;
+    }
+    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t s0 = blk * NT + threadIdx.x;
+    const bool valid = s0 < a.g.Vh;
+    const uint32_t s = valid ? s0 : 0;
+    R acc[NR][6];
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            acc[j][k] = 0;
+    if (valid) {
+        const SiteCoord sc = site_coords(a.g, s, a.dst_parity);
+        hop_fwd<A, LF, NR, 0, 1>(a, a.fat, s, sc, rr, acc);
+        hop_fwd<A, LL, NR, 0, 3>(a, a.lng, s, sc, rr, acc);
+        hop_fwd<A, LF, NR, 1, 1>(a, a.fat, s, sc, rr, acc);
+        hop_fwd<A, LL, NR, 1, 3>(a, a.lng, s, sc, rr, acc);
+        hop_fwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, rr, acc);
+        hop_fwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, rr, acc);
+        hop_fwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, rr, acc);
+        hop_fwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, rr, acc);
+        hop_bwd<A, LF, NR, 0, 1>(a, a.fat, s, sc, rr, acc);
+        hop_bwd<A, LL, NR, 0, 3>(a, a.lng, s, sc, rr, acc);
+        hop_bwd<A, LF, NR, 1, 1>(a, a.fat, s, sc, rr, acc);
+        hop_bwd<A, LL, NR, 1, 3>(a, a.lng, s, sc, rr, acc);
+        hop_bwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, rr, acc);
+        hop_bwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, rr, acc);
+        hop_bwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, rr, acc);
+        hop_bwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, rr, acc);
+    }
+    if constexpr (EPI == EPI_STORE) {
+        if (valid) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                V2 o[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    o[c] = mk<V2>((SR)acc[j][2 * c], (SR)acc[j][2 * c + 1]);
+                // padding slots repeat the last live right-hand side, so the
+                // store is unconditional (same value to the same address)
+                a.dst.store(rr[j], s, o);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            const bool live = j0 + j < nact;  // uniform over the block
+            double d = 0.0;
+            if (valid && live) {
+                const double sg = a.sigma[rr[j]];
+                V2 p[3], o[3];
+                a.aux.load(rr[j], s, p);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const SR ox = (SR)((R)sg * (R)p[c].x - acc[j][2 * c]);
+                    const SR oy = (SR)((R)sg * (R)p[c].y - acc[j][2 * c + 1]);
+                    o[c] = mk<V2>(ox, oy);
+                    d += (double)p[c].x * (double)ox + (double)p[c].y * (double)oy;
+                }
+                a.dst.store(rr[j], s, o);
+            }
+            const double tot = block_sum<NT>(d, lds);
+            if (threadIdx.x == 0 && live)
+                a.partial[(size_t)rr[j] * a.nblk + blockIdx.x] = tot;
+            __syncthreads();
+        }
+    }
+}
+
+template <class LF, class LL, typename SR, int NR, int EPI>
+static void launch_nt(const DslashArgs<LF, LL, SR> &a, int nt, hipStream_t st)
+{
+    const int nact_max = a.nrhs;
+    dim3 grid((a.g.Vh + nt - 1) / nt, (nact_max + NR - 1) / NR);
+    switch (nt) {
+    case 64:
+        hipLaunchKernelGGL((k_dslash<LF, LL, SR, NR, EPI, 64>), grid, dim3(64), 0, st, a);
+        break;
+    case 128:
+        hipLaunchKernelGGL((k_dslash<LF, LL, SR, NR, EPI, 128>), grid, dim3(128), 0, st, a);
+        break;
+    case 256:
+        hipLaunchKernelGGL((k_dslash<LF, LL, SR, NR, EPI, 256>), grid, dim3(256), 0, st, a);
+        break;
+    default:
+        throw std::runtime_error("dslash: unsupported block size " + std::to_string(nt));
+    }
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+template <class LF, class LL, typename SR, int NR>
+static void launch_epi(const DslashArgs<LF, LL, SR> &a, int epi, int nt, hipStream_t st)
+{
+    if (epi == EPI_STORE)
+        launch_nt<LF, LL, SR, NR, EPI_STORE>(a, nt, st);
+    else
+        launch_nt<LF, LL, SR, NR, EPI_AXMY_DOT>(a, nt, st);
+}
+
+template <class LF, class LL, typename SR>
+void launch_dslash(DslashArgs<LF, LL, SR> a, int epi, int nr, int nt, hipStream_t st)
+{
+    if (a.g.Vh == 0 || a.nrhs <= 0)
+        return;
+    a.nblk = (a.g.Vh + nt - 1) / nt;
+    switch (nr) {
+    case 1:
+        launch_epi<LF, LL, SR, 1>(a, epi, nt, st);
+        break;
+    case 2:
+        launch_epi<LF, LL, SR, 2>(a, epi, nt, st);
+        break;
+    case 4:
+        launch_epi<LF, LL, SR, 4>(a, epi, nt, st);
+        break;
+    case 8:
+        launch_epi<LF, LL, SR, 8>(a, epi, nt, st);
+        break;
+    default:
+        throw std::runtime_error("dslash: unsupported RHS blocking " + std::to_string(nr));
+    }
+}
+
+template void launch_dslash<LinkD, LinkD, double>(DslashArgs<LinkD, LinkD, double>, int, int, int, hipStream_t);
+template void launch_dslash<LinkF, LinkF, float>(DslashArgs<LinkF, LinkF, float>, int, int, int, hipStream_t);
+template void launch_dslash<LinkH, LinkH, float>(DslashArgs<LinkH, LinkH, float>, int, int, int, hipStream_t);
+template void launch_dslash<LinkH, LinkH12, float>(DslashArgs<LinkH, LinkH12, float>, int, int, int, hipStream_t);
+
+}  // namespace kq

@@ -1,0 +1,246 @@
+// Layout and precision conversion kernels (run once per gauge load and per
+// solve for the source/solutions; not on the iteration path).
+#include "kq_layout.h"
+
+#include <algorithm>
+
+namespace kq {
+
+template <typename HR, typename LR>
+__global__ void k_gauge_host_to_soa(const HR *raw, size_t i0, size_t n, c2_t<LR> *dst, uint32_t Vh, double scale)
+{
+    // one thread per (site, mu, k) of the chunk; consecutive threads read consecutive complex entries
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * 36)
+        return;
+    const size_t site = t / 36;
+    const int rem = (int)(t - site * 36);
+    const int mu = rem / 9, k = rem - mu * 9;
+    const size_t i = i0 + site;
+    const uint32_t q = (uint32_t)(i / Vh);
+    const uint32_t s = (uint32_t)(i - (size_t)q * Vh);
+    const HR re = raw[2 * t], im = raw[2 * t + 1];
+    c2_t<LR> v;
+    v.x = (LR)((double)re * scale);
+    v.y = (LR)((double)im * scale);
+    dst[((size_t)(q * 4 + mu) * 9 + k) * Vh + s] = v;
+}
+
+template <typename HR, typename LR>
+void gauge_host_to_soa(const HR *raw, size_t i0, size_t n, c2_t<LR> *dst, uint32_t Vh, double scale,
+                       hipStream_t st)
+{
+    const size_t nt = n * 36;
+    if (nt == 0)
+        return;
+    hipLaunchKernelGGL((k_gauge_host_to_soa<HR, LR>), dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, raw, i0,
+                       n, dst, Vh, scale);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_soa_d_to_f(const double2 *src, float2 *dst, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = mk<float2>((float)src[i].x, (float)src[i].y);
+}
+
+void soa_d_to_f(const double2 *src, float2 *dst, size_t n, hipStream_t st)
+{
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_soa_d_to_f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// one thread per link (q, mu, s)
+__global__ void k_soa_d_to_h(const double2 *src, __half2 *dst, float *scale, uint32_t Vh)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)8 * Vh)
+        return;
+    const size_t qm = t / Vh, s = t - qm * Vh;
+    double amax = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        const double2 v = src[(qm * 9 + k) * Vh + s];
+        amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
+    }
+    const float a = amax > 0.0 ? (float)amax : 1.0f;
+    for (int k = 0; k < 9; ++k) {
+        const double2 v = src[(qm * 9 + k) * Vh + s];
+        dst[(qm * 9 + k) * Vh + s] = __floats2half2_rn((float)(v.x / a), (float)(v.y / a));
+    }
+    scale[qm * Vh + s] = a;
+}
+
+void soa_d_to_h(const double2 *src, __half2 *dst, float *scale, uint32_t Vh, hipStream_t st)
+{
+    const size_t n = (size_t)8 * Vh;
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_soa_d_to_h, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, scale, Vh);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// det of a 3x3 complex matrix m[row*3+col]
+__device__ inline double2 det3(const double2 m[9])
+{
+    auto mul = [](double2 a, double2 b) { return mk<double2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+    auto sub = [](double2 a, double2 b) { return mk<double2>(a.x - b.x, a.y - b.y); };
+    auto add = [](double2 a, double2 b) { return mk<double2>(a.x + b.x, a.y + b.y); };
+    const double2 c0 = sub(mul(m[4], m[8]), mul(m[5], m[7]));
+    const double2 c1 = sub(mul(m[5], m[6]), mul(m[3], m[8]));
+    const double2 c2 = sub(mul(m[3], m[7]), mul(m[4], m[6]));
+    return add(add(mul(m[0], c0), mul(m[1], c1)), mul(m[2], c2));
+}
+
+__global__ void k_soa_d_to_h12(const double2 *src, __half2 *dst, float *scale, uint32_t Vh, int *fail)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)8 * Vh)
+        return;
+    const size_t qm = t / Vh, s = t - qm * Vh;
+    double2 m[9];
+    for (int k = 0; k < 9; ++k)
+        m[k] = src[(qm * 9 + k) * Vh + s];
+    const double2 d = det3(m);
+    // b^3 = det(b V) must be real; b = cbrt(Re det)
+    const double b = cbrt(d.x);
+    bool ok = fabs(d.y) <= 1e-6 * fabs(d.x) && b != 0.0;
+    double2 v[9];
+    for (int k = 0; k < 9; ++k)
+        v[k] = ok ? mk<double2>(m[k].x / b, m[k].y / b) : mk<double2>(0.0, 0.0);
+    if (ok) {
+        // unitarity of V: rows orthonormal
+        for (int r1 = 0; r1 < 3 && ok; ++r1)
+            for (int r2 = r1; r2 < 3 && ok; ++r2) {
+                double re = 0.0, im = 0.0;
+                for (int c = 0; c < 3; ++c) {
+                    const double2 a = v[3 * r1 + c], bb = v[3 * r2 + c];
+                    re += a.x * bb.x + a.y * bb.y;
+                    im += a.y * bb.x - a.x * bb.y;
+                }
+                const double want = (r1 == r2) ? 1.0 : 0.0;
+                if (fabs(re - want) > 1e-6 || fabs(im) > 1e-6)
+                    ok = false;
+            }
+    }
+    if (!ok)
+        atomicAdd(fail, 1);
+    for (int k = 0; k < 6; ++k)
+        dst[(qm * 6 + k) * Vh + s] = __floats2half2_rn((float)v[k].x, (float)v[k].y);
+    scale[qm * Vh + s] = (float)b;
+}
+
+void soa_d_to_h12(const double2 *src, __half2 *dst, float *scale, uint32_t Vh, int *fail, hipStream_t st)
+{
+    const size_t n = (size_t)8 * Vh;
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_soa_d_to_h12, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, scale, Vh,
+                       fail);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+template <typename HR, typename VR>
+__global__ void k_vec_host_to_soa(const HR *raw, Vec<VR> v, int r)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.Vh)
+        return;
+    c2_t<VR> o[3];
+    for (int c = 0; c < 3; ++c) {
+        o[c].x = (VR)raw[6 * (size_t)s + 2 * c];
+        o[c].y = (VR)raw[6 * (size_t)s + 2 * c + 1];
+    }
+    v.store(r, s, o);
+}
+
+template <typename HR, typename VR>
+void vec_host_to_soa(const HR *raw, Vec<VR> v, int r, hipStream_t st)
+{
+    if (v.Vh == 0)
+        return;
+    hipLaunchKernelGGL((k_vec_host_to_soa<HR, VR>), dim3((v.Vh + 255) / 256), dim3(256), 0, st, raw, v, r);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+template <typename VR, typename HR>
+__global__ void k_vec_soa_to_host(Vec<VR> v, int r, HR *raw)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.Vh)
+        return;
+    c2_t<VR> o[3];
+    v.load(r, s, o);
+    for (int c = 0; c < 3; ++c) {
+        raw[6 * (size_t)s + 2 * c] = (HR)o[c].x;
+        raw[6 * (size_t)s + 2 * c + 1] = (HR)o[c].y;
+    }
+}
+
+template <typename VR, typename HR>
+void vec_soa_to_host(Vec<VR> v, int r, HR *raw, hipStream_t st)
+{
+    if (v.Vh == 0)
+        return;
+    hipLaunchKernelGGL((k_vec_soa_to_host<VR, HR>), dim3((v.Vh + 255) / 256), dim3(256), 0, st, v, r, raw);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+template <typename XR, typename YR>
+__global__ void k_convert_copy(const c2_t<XR> *x, c2_t<YR> *y, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        c2_t<YR> o;
+        o.x = (YR)x[i].x;
+        o.y = (YR)x[i].y;
+        y[i] = o;
+    }
+}
+
+template <typename XR, typename YR>
+void convert_copy(const c2_t<XR> *x, c2_t<YR> *y, size_t n, hipStream_t st)
+{
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL((k_convert_copy<XR, YR>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, y, n);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        b[i] = a[i];
+}
+
+void device_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
+{
+    const size_t n = bytes / 16;
+    if (n == 0)
+        return;
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, st, static_cast<const uint4 *>(src), static_cast<uint4 *>(dst),
+                       n);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+template void gauge_host_to_soa<double, double>(const double *, size_t, size_t, double2 *, uint32_t, double,
+                                                hipStream_t);
+template void gauge_host_to_soa<float, float>(const float *, size_t, size_t, float2 *, uint32_t, double, hipStream_t);
+template void gauge_host_to_soa<float, double>(const float *, size_t, size_t, double2 *, uint32_t, double,
+                                               hipStream_t);
+template void vec_host_to_soa<double, double>(const double *, Vec<double>, int, hipStream_t);
+template void vec_host_to_soa<float, float>(const float *, Vec<float>, int, hipStream_t);
+template void vec_host_to_soa<double, float>(const double *, Vec<float>, int, hipStream_t);
+template void vec_soa_to_host<double, double>(Vec<double>, int, double *, hipStream_t);
+template void vec_soa_to_host<float, float>(Vec<float>, int, float *, hipStream_t);
+template void vec_soa_to_host<float, double>(Vec<float>, int, double *, hipStream_t);
+template void convert_copy<double, float>(const double2 *, float2 *, size_t, hipStream_t);
+template void convert_copy<float, double>(const float2 *, double2 *, size_t, hipStream_t);
+template void convert_copy<double, double>(const double2 *, double2 *, size_t, hipStream_t);
+template void convert_copy<float, float>(const float2 *, float2 *, size_t, hipStream_t);
+
+}  // namespace kq

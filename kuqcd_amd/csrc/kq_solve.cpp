@@ -1,0 +1,400 @@
+// Solve drivers: the reference multishift CG in the loaded precision, and the
+// FP64 / mixed-precision multishift CG with per-shift FP64 verification and
+// batched refinement.
+#include "kq_solve.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace kq {
+
+using clk = std::chrono::steady_clock;
+static double secs(clk::time_point a, clk::time_point b)
+{
+    return std::chrono::duration<double>(b - a).count();
+}
+
+std::vector<double> reduce_host(const double *part_dev, uint32_t nblk, int n, hipStream_t st)
+{
+    DevBuf out(sizeof(double) * n);
+    reduce_partials(part_dev, nblk, n, out.as<double>(), st);
+    std::vector<double> h(n);
+    KQ_HIP_CHECK(hipMemcpyAsync(h.data(), out.as<void>(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    KQ_HIP_CHECK(hipStreamSynchronize(st));
+    return h;
+}
+
+// host vector (reference layout, HR) -> device SoA vector r of v
+template <typename HR, typename VR> static void upload_vec(const void *h, Vec<VR> v, int r, DevBuf &stage, hipStream_t st)
+{
+    const size_t n = (size_t)6 * v.Vh;
+    if (stage.bytes() < sizeof(HR) * n)
+        stage.alloc(sizeof(HR) * n);
+    KQ_HIP_CHECK(hipMemcpyAsync(stage.as<void>(), h, sizeof(HR) * n, hipMemcpyHostToDevice, st));
+    vec_host_to_soa<HR, VR>(stage.as<HR>(), v, r, st);
+    KQ_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+template <typename VR, typename HR> static void download_vec(Vec<VR> v, int r, void *h, DevBuf &stage, hipStream_t st)
+{
+    const size_t n = (size_t)6 * v.Vh;
+    if (stage.bytes() < sizeof(HR) * n)
+        stage.alloc(sizeof(HR) * n);
+    vec_soa_to_host<VR, HR>(v, r, stage.as<HR>(), st);
+    KQ_HIP_CHECK(hipMemcpyAsync(h, stage.as<void>(), sizeof(HR) * n, hipMemcpyDeviceToHost, st));
+    KQ_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+static void print_iters(int iter, int max_iter)
+{
+    // inverter_cgm.cpp:80-84
+    if (iter >= max_iter - 1)
+        log_info("CGM: Warning max iteration reached %d\n", iter);
+    else
+        log_info("CGM: # iterations %d\n", iter);
+}
+
+// ---------------------------------------------------------------- reference method
+template <typename T, class LK>
+static KuQCDStatus solve_ref_t(int slot, LK fat, LK lng, void **x, void *b, KuQCDInvertParam &p)
+{
+    Context &c = ctx();
+    const Geom g = slot_geom(slot);
+    const int parity = (p.matpc_type == KUQCD_EVEN_EVEN) ? 0 : 1;
+    const int ns = p.num_shift;
+    RunnerTuning tune;
+    const auto nrnt = tuned_launch(slot, KQ_MODE_REFERENCE, 1);
+    tune.nr = 1;
+    tune.nt = nrnt.second;
+    CgmRunner<T, T, LK, LK> run(g, parity, fat, lng, 1, ns, c.stream, tune);
+    DevBuf stage;
+    const size_t n3 = (size_t)3 * g.Vh;
+    Vec<T> rv{reinterpret_cast<c2_t<T> *>(run.r_buf()), g.Vh, n3};
+    upload_vec<T, T>(b, rv, 0, stage, c.stream);
+    std::vector<T> sh(ns);
+    for (int k = 0; k < ns; ++k)
+        sh[k] = (T)p.shifts[k];  // kuqcd.cpp:270-272
+    run.start(sh, std::vector<double>{p.tol}, p.max_iter);
+    run.run();
+    const int iter = run.states()[0].iter;
+    for (int k = 0; k < ns; ++k) {
+        Vec<T> xv{reinterpret_cast<c2_t<T> *>(run.x_buf()) + (size_t)k * n3, g.Vh, n3};
+        download_vec<T, T>(xv, 0, x[k], stage, c.stream);
+    }
+    p.iter = iter;
+    print_iters(iter, p.max_iter);
+    return KUQCD_SUCCESS;
+}
+
+KuQCDStatus solve_reference(int slot, void **x, void *b, KuQCDInvertParam &p)
+{
+    Context &c = ctx();
+    if (c.fat[slot].prec == KUQCD_DOUBLE)
+        return solve_ref_t<double, LinkD>(slot, link_d(c.fat[slot]), link_d(c.lng[slot]), x, b, p);
+    return solve_ref_t<float, LinkF>(slot, link_f(c.fat[slot]), link_f(c.lng[slot]), x, b, p);
+}
+
+// ---------------------------------------------------------------- FP64 verification
+// res2[k] = |b - A_k x_k|^2 for the nk vectors x[k] (FP64 SoA, stride 3Vh),
+// A_k = sigma_k - D D with absolute shifts sigma_k.
+static std::vector<double> true_res2(int slot, int parity, const double *b, const double *x, int nk,
+                                     const std::vector<double> &sigma_abs)
+{
+    Context &c = ctx();
+    const Geom g = slot_geom(slot);
+    const size_t n3 = (size_t)3 * g.Vh;
+    DevBuf tmp(sizeof(double2) * nk * n3), ax(sizeof(double2) * nk * n3), sig(sizeof(double) * nk);
+    const uint32_t nblk_d = dslash_nblk(g.Vh, 256);
+    DevBuf part(sizeof(double) * nk * std::max(nblk_d, blas_nblk(g.Vh)));
+    KQ_HIP_CHECK(hipMemcpyAsync(sig.as<void>(), sigma_abs.data(), sizeof(double) * nk, hipMemcpyHostToDevice,
+                                c.stream));
+    DslashArgs<LinkD, LinkD, double> a{};
+    a.g = g;
+    a.fat = link_d(c.fat[slot]);
+    a.lng = link_d(c.lng[slot]);
+    a.nrhs = nk;
+    Vec<double> xv{reinterpret_cast<double2 *>(const_cast<double *>(x)), g.Vh, n3};
+    Vec<double> tv{tmp.as<double2>(), g.Vh, n3}, av{ax.as<double2>(), g.Vh, n3};
+    a.dst_parity = 1 - parity;
+    a.src = xv;
+    a.dst = tv;
+    launch_dslash(a, EPI_STORE, 1, 256, c.stream);
+    a.dst_parity = parity;
+    a.src = tv;
+    a.dst = av;
+    a.aux = xv;
+    a.sigma = sig.as<double>();
+    a.partial = part.as<double>();
+    launch_dslash(a, EPI_AXMY_DOT, 1, 256, c.stream);
+    const uint32_t nb = blas_nblk(g.Vh);
+    resid2_partial(b, ax.as<double>(), nk, g.Vh, part.as<double>(), nb, c.stream);
+    return reduce_host(part.as<double>(), nb, nk, c.stream);
+}
+
+static std::vector<double> abs_shifts(const KuQCDInvertParam &p, int r)
+{
+    // Q2: system k >= 1 is K + shifts[0] + shifts[k] (inverter_cgm.cpp:52)
+    const double *s = p.shifts + (size_t)r * p.num_shift;
+    std::vector<double> a(p.num_shift);
+    for (int k = 0; k < p.num_shift; ++k)
+        a[k] = (k == 0) ? s[0] : s[0] + s[k];
+    return a;
+}
+
+// ---------------------------------------------------------------- mixed precision
+struct Problem {
+    int slot, parity, nrhs, ns;
+    Geom g;
+    const KuQCDInvertParam *p;
+    const KqSolveOptions *o;
+    DevBuf *b;  // FP64 [nrhs][3][Vh]
+    std::vector<double> b2, tol, eps;
+};
+
+template <class LF, class LL>
+static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *rep, std::vector<double> &res2)
+{
+    Context &c = ctx();
+    const size_t n3 = (size_t)3 * P.g.Vh;
+    const auto nrnt = tuned_launch(P.slot, P.o->mode, P.nrhs);
+    RunnerTuning tune;
+    tune.nr = P.o->nr > 0 ? P.o->nr : nrnt.first;
+    tune.nt = P.o->nt > 0 ? P.o->nt : nrnt.second;
+    tune.chunk = P.o->chunk > 0 ? P.o->chunk : 16;
+    const double delta = P.o->reliable_delta > 0 ? P.o->reliable_delta : 0.1;
+    const LinkD f64 = link_d(c.fat[P.slot]), l64 = link_d(c.lng[P.slot]);
+
+    auto t0 = clk::now();
+    {
+        CgmRunner<double, float, LF, LL> run(P.g, P.parity, fat, lng, P.nrhs, P.ns, c.stream, tune);
+        run.enable_reliable(f64, l64, delta);
+        KQ_HIP_CHECK(hipMemcpyAsync(run.b_hi(), P.b->as<void>(), sizeof(double2) * P.nrhs * n3,
+                                    hipMemcpyDeviceToDevice, c.stream));
+        std::vector<double> sh((size_t)P.nrhs * P.ns);
+        for (size_t i = 0; i < sh.size(); ++i)
+            sh[i] = P.p->shifts[i];
+        run.start(sh, P.tol, P.p->max_iter);
+        run.run();
+        const auto st = run.states();
+        for (int r = 0; r < P.nrhs; ++r) {
+            if (rep) {
+                rep[r].iter = st[r].iter;
+                rep[r].reliable_updates = run.reliable_updates();
+            }
+        }
+        x_out.alloc(sizeof(double2) * P.nrhs * P.ns * n3);
+        KQ_HIP_CHECK(hipMemcpyAsync(x_out.as<void>(), run.x_hi(), x_out.bytes(), hipMemcpyDeviceToDevice, c.stream));
+        KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+    }
+    auto t1 = clk::now();
+    if (rep)
+        for (int r = 0; r < P.nrhs; ++r)
+            rep[r].seconds_multishift = secs(t0, t1);
+
+    // per-shift FP64 verification and batched refinement
+    res2.assign((size_t)P.nrhs * P.ns, 0.0);
+    auto verify = [&]() {
+        for (int r = 0; r < P.nrhs; ++r) {
+            const auto v = true_res2(P.slot, P.parity, P.b->as<double>() + (size_t)r * 2 * n3,
+                                     x_out.as<double>() + (size_t)r * P.ns * 2 * n3, P.ns, abs_shifts(*P.p, r));
+            for (int k = 0; k < P.ns; ++k)
+                res2[(size_t)r * P.ns + k] = v[k];
+        }
+    };
+    verify();
+    if (!P.o->refine)
+        return;
+    for (int round = 0; round < 3; ++round) {
+        std::vector<std::pair<int, int>> need;
+        for (int r = 0; r < P.nrhs; ++r)
+            for (int k = 0; k < P.ns; ++k)
+                if (std::sqrt(res2[(size_t)r * P.ns + k] / P.b2[r]) > P.eps[r])
+                    need.emplace_back(r, k);
+        if (need.empty())
+            break;
+        const int nsys = (int)need.size();
+        RunnerTuning t2 = tune;
+        t2.nr = P.o->nr > 0 ? P.o->nr : tuned_launch(P.slot, P.o->mode, nsys).first;
+        CgmRunner<double, float, LF, LL> ref(P.g, P.parity, fat, lng, nsys, 1, c.stream, t2);
+        ref.enable_reliable(f64, l64, delta);
+        std::vector<double> sh(nsys), tol(nsys);
+        for (int i = 0; i < nsys; ++i) {
+            const int r = need[i].first, k = need[i].second;
+            KQ_HIP_CHECK(hipMemcpyAsync(ref.b_hi() + (size_t)i * 2 * n3, P.b->as<double>() + (size_t)r * 2 * n3,
+                                        sizeof(double2) * n3, hipMemcpyDeviceToDevice, c.stream));
+            KQ_HIP_CHECK(hipMemcpyAsync(ref.x_hi() + (size_t)i * 2 * n3,
+                                        x_out.as<double>() + ((size_t)r * P.ns + k) * 2 * n3, sizeof(double2) * n3,
+                                        hipMemcpyDeviceToDevice, c.stream));
+            sh[i] = abs_shifts(*P.p, r)[k];
+            tol[i] = 0.25 * P.eps[r] * P.eps[r] * P.b2[r];
+        }
+        ref.start(sh, tol, P.p->max_iter, true);
+        ref.run();
+        const auto st = ref.states();
+        for (int i = 0; i < nsys; ++i) {
+            const int r = need[i].first, k = need[i].second;
+            KQ_HIP_CHECK(hipMemcpyAsync(x_out.as<double>() + ((size_t)r * P.ns + k) * 2 * n3,
+                                        ref.x_hi() + (size_t)i * 2 * n3, sizeof(double2) * n3,
+                                        hipMemcpyDeviceToDevice, c.stream));
+            if (rep) {
+                rep[r].refine_iter[k] += st[i].iter;
+                rep[r].refine_rounds = round + 1;
+            }
+        }
+        KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+        verify();
+    }
+    auto t2 = clk::now();
+    if (rep)
+        for (int r = 0; r < P.nrhs; ++r)
+            rep[r].seconds_refine = secs(t1, t2);
+}
+
+static void run_fp64(Problem &P, DevBuf &x_out, KqSolveReport *rep, std::vector<double> &res2)
+{
+    Context &c = ctx();
+    const size_t n3 = (size_t)3 * P.g.Vh;
+    const auto nrnt = tuned_launch(P.slot, KQ_MODE_FP64, P.nrhs);
+    RunnerTuning tune;
+    tune.nr = P.o->nr > 0 ? P.o->nr : nrnt.first;
+    tune.nt = P.o->nt > 0 ? P.o->nt : nrnt.second;
+    tune.chunk = P.o->chunk > 0 ? P.o->chunk : 16;
+    auto t0 = clk::now();
+    CgmRunner<double, double, LinkD, LinkD> run(P.g, P.parity, link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.nrhs,
+                                                P.ns, c.stream, tune);
+    KQ_HIP_CHECK(hipMemcpyAsync(run.r_buf(), P.b->as<void>(), sizeof(double2) * P.nrhs * n3, hipMemcpyDeviceToDevice,
+                                c.stream));
+    std::vector<double> sh((size_t)P.nrhs * P.ns);
+    for (size_t i = 0; i < sh.size(); ++i)
+        sh[i] = P.p->shifts[i];
+    run.start(sh, P.tol, P.p->max_iter);
+    run.run();
+    const auto st = run.states();
+    x_out.alloc(sizeof(double2) * P.nrhs * P.ns * n3);
+    KQ_HIP_CHECK(hipMemcpyAsync(x_out.as<void>(), run.x_buf(), x_out.bytes(), hipMemcpyDeviceToDevice, c.stream));
+    KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+    auto t1 = clk::now();
+    if (rep)
+        for (int r = 0; r < P.nrhs; ++r) {
+            rep[r].iter = st[r].iter;
+            rep[r].seconds_multishift = secs(t0, t1);
+        }
+    res2.assign((size_t)P.nrhs * P.ns, 0.0);
+    if (P.o->refine)
+        for (int r = 0; r < P.nrhs; ++r) {
+            const auto v = true_res2(P.slot, P.parity, P.b->as<double>() + (size_t)r * 2 * n3,
+                                     x_out.as<double>() + (size_t)r * P.ns * 2 * n3, P.ns, abs_shifts(*P.p, r));
+            for (int k = 0; k < P.ns; ++k)
+                res2[(size_t)r * P.ns + k] = v[k];
+        }
+}
+
+KuQCDStatus solve_extended(int slot, int nrhs, void **x, void **b, KuQCDInvertParam &p, const KqSolveOptions &o,
+                           KqSolveReport *rep)
+{
+    auto t0 = clk::now();
+    Context &c = ctx();
+    std::vector<KqSolveReport> local(rep ? 0 : nrhs);
+    if (!rep)
+        rep = local.data();
+    for (int r = 0; r < nrhs; ++r)
+        std::memset(&rep[r], 0, sizeof(KqSolveReport));
+    if (o.mode == KQ_MODE_REFERENCE && c.fat[slot].prec != KUQCD_DOUBLE) {
+        if (nrhs != 1) {
+            set_error("batched solves need a double-precision gauge field");
+            return KUQCD_ERR_UNSUPPORTED;
+        }
+        KuQCDStatus st = solve_reference(slot, x, b[0], p);
+        if (rep) {
+            rep[0].iter = p.iter;
+            rep[0].seconds = secs(t0, clk::now());
+        }
+        return st;
+    }
+    if (c.fat[slot].prec != KUQCD_DOUBLE) {
+        set_error("the FP64 and mixed-precision modes need a double-precision gauge field");
+        return KUQCD_ERR_UNSUPPORTED;
+    }
+    if (o.learned) {
+        set_error("the learned refinement is not available in this build");
+        return KUQCD_ERR_UNSUPPORTED;
+    }
+    Problem P;
+    P.slot = slot;
+    P.parity = (p.matpc_type == KUQCD_EVEN_EVEN) ? 0 : 1;
+    P.nrhs = nrhs;
+    P.ns = p.num_shift;
+    P.g = slot_geom(slot);
+    P.p = &p;
+    P.o = &o;
+    const size_t n3 = (size_t)3 * P.g.Vh;
+    DevBuf bdev(sizeof(double2) * nrhs * n3), stage;
+    for (int r = 0; r < nrhs; ++r) {
+        Vec<double> bv{bdev.as<double2>(), P.g.Vh, n3};
+        upload_vec<double, double>(b[r], bv, r, stage, c.stream);
+    }
+    P.b = &bdev;
+    {
+        DevBuf part(sizeof(double) * nrhs * blas_nblk(P.g.Vh));
+        Vec<double> bv{bdev.as<double2>(), P.g.Vh, n3};
+        vec_dot_partial<double>(bv, bv, nrhs, part.as<double>(), blas_nblk(P.g.Vh), c.stream);
+        P.b2 = reduce_host(part.as<double>(), blas_nblk(P.g.Vh), nrhs, c.stream);
+    }
+    P.tol.resize(nrhs);
+    P.eps.resize(nrhs);
+    for (int r = 0; r < nrhs; ++r) {
+        if (o.rel_tol > 0) {
+            P.tol[r] = o.rel_tol * o.rel_tol * P.b2[r];
+            P.eps[r] = o.rel_tol;
+        } else {
+            P.tol[r] = p.tol;
+            P.eps[r] = std::sqrt(p.tol / P.b2[r]);
+        }
+    }
+    DevBuf xdev;
+    std::vector<double> res2;
+    const int mode = (o.mode == KQ_MODE_REFERENCE) ? KQ_MODE_FP64 : o.mode;
+    if (mode == KQ_MODE_FP64) {
+        run_fp64(P, xdev, rep, res2);
+    } else if (mode == KQ_MODE_MIXED_FP32) {
+        DerivedFormats &d = derived(slot, true, false);
+        run_mixed<LinkF, LinkF>(P, LinkF{d.fat_f.as<float2>(), P.g.Vh}, LinkF{d.lng_f.as<float2>(), P.g.Vh}, xdev,
+                                rep, res2);
+    } else if (mode == KQ_MODE_MIXED_FP16) {
+        DerivedFormats &d = derived(slot, false, true);
+        LinkH fh{d.fat_h.as<__half2>(), d.fat_hs.as<float>(), P.g.Vh};
+        if (d.long12)
+            run_mixed<LinkH, LinkH12>(P, fh, LinkH12{d.lng_h.as<__half2>(), d.lng_hs.as<float>(), P.g.Vh}, xdev, rep,
+                                      res2);
+        else
+            run_mixed<LinkH, LinkH>(P, fh, LinkH{d.lng_h.as<__half2>(), d.lng_hs.as<float>(), P.g.Vh}, xdev, rep,
+                                    res2);
+    } else {
+        set_error("unknown mode");
+        return KUQCD_ERR_INVALID_PARAM;
+    }
+    // solutions to the host (double)
+    for (int r = 0; r < nrhs; ++r)
+        for (int k = 0; k < P.ns; ++k) {
+            Vec<double> xv{xdev.as<double2>() + ((size_t)r * P.ns + k) * n3, P.g.Vh, n3};
+            download_vec<double, double>(xv, 0, x[(size_t)r * P.ns + k], stage, c.stream);
+        }
+    const double total = secs(t0, clk::now());
+    int maxit = 0;
+    for (int r = 0; r < nrhs; ++r) {
+        if (rep) {
+            for (int k = 0; k < P.ns; ++k)
+                rep[r].true_rel_res[k] = std::sqrt(res2[(size_t)r * P.ns + k] / P.b2[r]);
+            rep[r].seconds = total;
+            maxit = std::max(maxit, rep[r].iter);
+        }
+    }
+    p.iter = maxit;
+    if (o.verbose > 0)
+        print_iters(maxit, p.max_iter);
+    return KUQCD_SUCCESS;
+}
+
+}  // namespace kq

@@ -1,0 +1,112 @@
+"""Parity of the HIP hopping term with the oracle (dirac_hisq.cpp:409-502).
+
+Tolerances (relative 2-norm over the whole output vector):
+  FP64 links/vectors            <= 1e-13
+  FP32 links/vectors            <= 1e-5
+  FP16 links (+FP32 scale), FP32 vectors  <= 3e-3 (half precision has an 11-bit significand)
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+
+LATS = [(4, 4, 4, 4), (8, 8, 8, 8), (6, 4, 2, 8), (4, 4, 4, 1), (2, 2, 2, 2), (4, 6, 1, 3)]
+
+
+def _setup(kq, lat, prec, kind=0, seed=11):
+    fat, lng = kq.gen_gauge(lat, kind=kind, seed_fat=seed, seed_long=seed + 1)
+    if prec == kq.KUQCD_SINGLE:
+        fat = fat.astype(np.float32)
+        lng = lng.astype(np.float32)
+    kq.load_gauge_pair(fat, lng, lat, prec=prec)
+    return fat, lng
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lat", LATS)
+@pytest.mark.parametrize("parity", [0, 1])
+def test_dslash_fp64(gpu, lat, parity):
+    print("case", lat, parity, flush=True)
+    fat, lng = _setup(gpu, lat, gpu.KUQCD_DOUBLE)
+    src = gpu.gen_source(lat, 1 - parity, seed=5)
+    dst = np.empty_like(src)
+    assert gpu.dslash(src, dst, parity) == 0, gpu.last_error()
+    ref = oracle.dslash(fat, lng, src, lat, parity)
+    assert rel_err(dst, ref) <= 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lat", [(4, 4, 4, 4), (8, 8, 8, 8), (4, 4, 4, 1)])
+def test_dslash_fp32_gauge(gpu, lat):
+    fat, lng = _setup(gpu, lat, gpu.KUQCD_SINGLE)
+    src = gpu.gen_source(lat, 1, seed=6).astype(np.float32)
+    dst = np.empty_like(src)
+    assert gpu.dslash(src, dst, 0) == 0, gpu.last_error()
+    ref32 = oracle.dslash(fat, lng, src, lat, 0)
+    ref64 = oracle.dslash(fat.astype(np.float64), lng.astype(np.float64), src.astype(np.float64), lat, 0)
+    assert rel_err(dst, ref32) <= 1e-5
+    assert rel_err(dst, ref64) <= 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,tol", [(2, 1e-5), (3, 3e-3)])
+@pytest.mark.parametrize("kind", [0, 1])
+def test_dslash_lower_precision_modes(gpu, mode, tol, kind):
+    lat = (8, 8, 8, 8)
+    fat, lng = _setup(gpu, lat, gpu.KUQCD_DOUBLE, kind=kind)
+    src = gpu.gen_source(lat, 0, seed=7)
+    dst = np.empty_like(src)
+    assert gpu.dslash(src, dst, 1, mode) == 0, gpu.last_error()
+    ref = oracle.dslash(fat, lng, src, lat, 1)
+    assert rel_err(dst, ref) <= tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_dslash_batched_equals_single(gpu, mode):
+    lat = (8, 4, 4, 8)
+    _setup(gpu, lat, gpu.KUQCD_DOUBLE)
+    nrhs = 5
+    src = np.stack([gpu.gen_source(lat, 1, seed=100 + r) for r in range(nrhs)])
+    single = np.empty_like(src)
+    for r in range(nrhs):
+        assert gpu.dslash(np.ascontiguousarray(src[r]), single[r], 0, mode) == 0
+    nrs = [1, 2] if mode == 1 else [1, 2, 4, 8]
+    for nr in nrs:
+        for nt in (64, 128, 256):
+            out = np.empty_like(src)
+            assert gpu.dslash_batched(src, out, 0, mode, nr, nt) == 0, gpu.last_error()
+            # same arithmetic per right-hand side: bit-identical
+            assert np.array_equal(out, single), (nr, nt)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("parity", [0, 1])
+def test_mdaggm_fp64(gpu, parity):
+    lat = (8, 8, 8, 8)
+    fat, lng = _setup(gpu, lat, gpu.KUQCD_DOUBLE)
+    src = gpu.gen_source(lat, parity, seed=8)
+    dst = np.empty_like(src)
+    sigma = 0.0123
+    assert gpu.mdaggm(src, dst, parity, sigma) == 0
+    ref = sigma * src - oracle.mdaggm(fat, lng, src, lat, parity)
+    assert rel_err(dst, ref) <= 1e-13
+
+
+@pytest.mark.gpu
+def test_dslash_antihermitian_full_size(gpu):
+    """<u, D_{p,pbar} v> = -<D_{pbar,p} u, v> at the benchmark lattice 32^4 (size-independent property)."""
+    lat = (32, 32, 32, 32)
+    _setup(gpu, lat, gpu.KUQCD_DOUBLE, kind=1)
+    u = gpu.gen_source(lat, 0, seed=21)
+    v = gpu.gen_source(lat, 1, seed=22)
+    Dv = np.empty_like(u)
+    Du = np.empty_like(v)
+    assert gpu.dslash(v, Dv, 0) == 0
+    assert gpu.dslash(u, Du, 1) == 0
+    cu = u[..., 0] + 1j * u[..., 1]
+    cv = v[..., 0] + 1j * v[..., 1]
+    a = np.vdot(cu, Dv[..., 0] + 1j * Dv[..., 1])
+    b = np.vdot(Du[..., 0] + 1j * Du[..., 1], cv)
+    assert abs(a + b) <= 1e-12 * abs(a)
