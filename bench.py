@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the HISQ even-odd multishift CG on one MI355X.
+
+Workload (synthetic, seeded): 32^4 periodic lattice, fat links 0.9 * exp(i 0.3 H)
+and long links -1/24 * W(x)W(x+mu)W(x+2mu) with W = exp(i 0.3 H), staggered
+phases folded in, Gaussian even-site source, mass m = 0.01, 12 shifts
+log-spaced in [1e-4, 1] (absolute shifts m^2 + s_k), FP64 true relative
+residual <= 1e-10 for every shift.
+
+One step = one complete mixed-precision multishift solve (FP32 links and
+vectors, FP64 reliable updates and recurrences, per-shift FP64 verification and
+refinement) with the source and the solutions resident in HBM. The line also
+carries the plain FP64 GPU multishift CG time on the same problem, the FP64
+hopping-term roofline, and the oracle CGM timed on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "12-shift CGM time to rel. residual 1e-10, 32^4: mixed vs FP64; dslash %HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pmc_traffic(path: Path):
+    """HBM bytes per launch of the FP64 hopping term from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    if not path.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", path.read_text())
+    return float(m.group(1)) if m else None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--lattice", default="32,32,32,32")
+    ap.add_argument("--nshift", type=int, default=12)
+    ap.add_argument("--mass", type=float, default=0.01)
+    ap.add_argument("--rel-tol", type=float, default=1e-10)
+    ap.add_argument("--mode", type=int, default=2, help="2 = mixed FP32, 3 = mixed FP16")
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build has no multi-GPU path "
+              "(the lattice would be split by 4-D domain decomposition with halo exchange; see DESIGN.md).",
+              file=sys.stderr)
+        return 2
+
+    import kuqcd_amd.api as A
+
+    A.set_verbosity(0)
+    assert A.kuqcd_init() == A.KUQCD_SUCCESS
+    lat = tuple(int(v) for v in args.lattice.split(","))
+    Vh = int(np.prod(lat)) // 2
+
+    t = time.time()
+    fat, lng = A.gen_gauge(lat, kind=A.KQ_GEN_WEAK, eps=0.3, fat_scale=0.9, c_naik=-1.0 / 24.0, seed_fat=1,
+                           seed_long=2)
+    b = A.gen_source(lat, 0, seed=3)
+    log(f"inputs generated in {time.time() - t:.1f}s")
+    A.load_gauge_pair(fat, lng, lat)
+
+    s = np.logspace(-4, 0, args.nshift)
+    m2 = args.mass ** 2
+    shifts = np.concatenate([[m2 + s[0]], s[1:] - s[0]])  # reference convention (shifts[k>=1] relative)
+    b2 = float(np.sum(b * b))
+    tol = args.rel_tol ** 2 * b2
+
+    # source and solutions resident in HBM (reference host layout, device memory)
+    L = A.L.lib()
+    nbytes = b.nbytes
+    b_dev = L.kq_device_alloc(nbytes)
+    x_dev = [L.kq_device_alloc(nbytes) for _ in range(args.nshift)]
+    assert b_dev and all(x_dev)
+    assert L.kq_memcpy(b_dev, b.ctypes.data, nbytes) == 0
+    import ctypes as C
+    xp = (C.c_void_p * args.nshift)(*x_dev)
+
+    def solve(mode):
+        prm = A.InvertParam(shifts=shifts, tol=tol, max_iter=40000)
+        opt = A.default_options(mode=mode, refine=1, rel_tol=args.rel_tol, verbose=0)
+        rep = A.L.KqSolveReport()
+        st = L.kq_solve_multishift_ex(xp, C.c_void_p(b_dev), C.byref(prm.struct()), C.byref(opt), C.byref(rep))
+        if st != A.KUQCD_SUCCESS:
+            raise RuntimeError(f"solve failed: {st} {A.last_error()}")
+        return rep
+
+    for w in range(args.warmup):
+        rep = solve(args.mode)
+        log(f"warmup {w}: {rep.seconds:.3f}s iter {rep.iter}")
+    L.kq_end()
+    t0 = time.perf_counter()
+    reps = []
+    for k in range(args.steps):
+        reps.append(solve(args.mode))
+        log(f"step {k}: {reps[-1].seconds:.3f}s iter {reps[-1].iter} refine {list(reps[-1].refine_iter[:args.nshift])}")
+    L.kq_end()
+    t1 = time.perf_counter()
+    sec_mixed = (t1 - t0) / args.steps
+    rep = reps[-1]
+    res_mixed = [rep.true_rel_res[k] for k in range(args.nshift)]
+
+    # plain FP64 GPU multishift CG on the same problem
+    solve(1)  # warm (autotune / derived formats)
+    t2 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        rep64 = solve(1)
+    L.kq_end()
+    t3 = time.perf_counter()
+    sec_fp64 = (t3 - t2) / args.fp64_steps
+    res_fp64 = [rep64.true_rel_res[k] for k in range(args.nshift)]
+    log(f"fp64: {sec_fp64:.3f}s iter {rep64.iter}; mixed {sec_mixed:.3f}s")
+
+    # hopping-term roofline (HIP events on the library stream)
+    rl = {}
+    for mode, name in ((1, "fp64"), (2, "fp32"), (3, "fp16")):
+        st, r = A.bench_dslash(mode, 1, 0, 0, 50)
+        assert st == 0, A.last_error()
+        rl[name] = r
+        log(f"dslash {name}: {r.ms * 1e3:.1f} us {r.gbs:.0f} GB/s {100 * r.hbm_frac:.1f}% nt {r.nt}")
+    st, cp = A.bench_copy(1 << 30, 20)
+    r64 = rl["fp64"]
+    traffic = pmc_traffic(ROOT / "profiles" / "dslash_fp64_32x32x32x32_pmc.txt")
+
+    # CPU baseline: the oracle CGM (same algorithm) on the host cores, bounded sample
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        nthreads = oracle.num_threads()
+        bo = np.ascontiguousarray(b)
+        t = time.perf_counter()
+        oracle.cgm(fat, lng, lat, 0, shifts, bo, tol, 3)
+        t_cal = (time.perf_counter() - t) / 3
+        n_it = int(max(3, min(2000, args.cpu_seconds / max(t_cal, 1e-6))))
+        t = time.perf_counter()
+        _, it_done = oracle.cgm(fat, lng, lat, 0, shifts, bo, tol, n_it)
+        t_it = (time.perf_counter() - t) / it_done
+        cpu = {
+            "value": round(t_it * rep64.iter, 3),
+            "unit": "s",
+            "cores": nthreads,
+            "kind": "port",
+            "sample": (f"oracle C restatement of CGM::Invert, FP64, same 32^4 12-shift problem, first {it_done} "
+                       f"iterations timed ({t_it * 1e3:.1f} ms/iteration with all 12 shifts active) x the "
+                       f"{rep64.iter} iterations of the FP64 solve"),
+        }
+        log(f"cpu: {t_it * 1e3:.1f} ms/iter x {rep64.iter} = {cpu['value']:.1f}s ({nthreads} threads)")
+
+    line = {
+        "metric": METRIC,
+        "value": round(sec_mixed, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec_mixed * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if args.mode == 2 else "fp16+fp32+fp64",
+        "data": "synthetic (seeded weak-field SU(3) links, Gaussian source)",
+        "config": {"workload": f"HISQ even-odd multishift CG, {'x'.join(map(str, lat))}, {args.nshift} shifts "
+                               f"log-spaced 1e-4..1, m={args.mass}, FP64 true rel. residual {args.rel_tol:g} per shift",
+                   "lattice": list(lat), "nshift": args.nshift, "mass": args.mass,
+                   "mode": "mixed-fp32" if args.mode == 2 else "mixed-fp16", "parallelism": "single GPU"},
+        "fp64_seconds": round(sec_fp64, 4),
+        "speedup_vs_fp64": round(sec_fp64 / sec_mixed, 3),
+        "iterations": {"mixed_multishift": rep.iter, "mixed_reliable_updates": rep.reliable_updates,
+                       "mixed_refine": [rep.refine_iter[k] for k in range(args.nshift)], "fp64": rep64.iter},
+        "max_true_rel_res": {"mixed": max(res_mixed), "fp64": max(res_fp64)},
+        "roofline": {"bound": "hbm", "achieved": round(r64.gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r64.gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_dslash<LinkD,LinkD,double> (FP64 hopping term, 2400 B/site)",
+                     "us_per_launch": round(r64.ms * 1e3, 2), "gflops": round(r64.gflops, 1)},
+        "roofline_fp32": {"achieved": round(rl["fp32"].gbs, 1), "frac": round(rl["fp32"].hbm_frac, 4),
+                          "us_per_launch": round(rl["fp32"].ms * 1e3, 2), "bytes_per_site": 1200},
+        "roofline_fp16": {"achieved": round(rl["fp16"].gbs, 1), "frac": round(rl["fp16"].hbm_frac, 4),
+                          "us_per_launch": round(rl["fp16"].ms * 1e3, 2),
+                          "bytes_per_site": round(rl["fp16"].bytes / Vh, 1)},
+        "copy_kernel_gbs": round(cp.gbs, 1),
+        "cpu_baseline": cpu,
+    }
+    for p in [b_dev, *x_dev]:
+        L.kq_device_free(p)
+    A.kuqcd_gauge_free()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

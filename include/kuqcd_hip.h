@@ -117,6 +117,13 @@ KQ_API int kq_gen_gauge(double *fat, double *lng, const unsigned lat[4], int kin
 /* complex Gaussian source, E|z|^2 = 1, on the Vh sites of `parity` */
 KQ_API int kq_gen_source(double *b, const unsigned lat[4], int parity, uint64_t seed);
 
+/* ---- device memory for callers that keep b and x on the GPU: every entry
+ * point above accepts host or device pointers (reference layout either way) */
+KQ_API void *kq_device_alloc(size_t bytes);
+KQ_API void kq_device_free(void *p);
+/* copy `bytes` between any two host/device pointers; returns 0 on success */
+KQ_API int kq_memcpy(void *dst, const void *src, size_t bytes);
+
 /* ---- misc */
 KQ_API const char *kq_last_error(void);
 KQ_API void kq_set_verbosity(int v);

@@ -128,6 +128,9 @@ _PROTOS = {
     "kq_gen_gauge": (C.c_int, [_VP, _VP, C.POINTER(C.c_uint), C.c_int, C.c_double, C.c_double, C.c_double,
                                C.c_uint64, C.c_uint64, C.c_int]),
     "kq_gen_source": (C.c_int, [_VP, C.POINTER(C.c_uint), C.c_int, C.c_uint64]),
+    "kq_device_alloc": (_VP, [C.c_size_t]),
+    "kq_device_free": (None, [_VP]),
+    "kq_memcpy": (C.c_int, [_VP, _VP, C.c_size_t]),
     "kq_last_error": (C.c_char_p, []),
     "kq_set_verbosity": (None, [C.c_int]),
     "kq_device_count": (C.c_int, []),

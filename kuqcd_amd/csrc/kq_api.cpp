@@ -204,7 +204,7 @@ static void upload_links(const HR *h, GaugeSlot &slot, double scale, int prec)
     DevBuf stage(sizeof(HR) * 72 * chunk);
     for (uint64_t i0 = 0; i0 < V; i0 += chunk) {
         const size_t n = (size_t)std::min<uint64_t>(chunk, V - i0);
-        KQ_HIP_CHECK(hipMemcpyAsync(stage.as<void>(), h + 72 * i0, sizeof(HR) * 72 * n, hipMemcpyHostToDevice,
+        KQ_HIP_CHECK(hipMemcpyAsync(stage.as<void>(), h + 72 * i0, sizeof(HR) * 72 * n, hipMemcpyDefault,
                                     c.stream));
         if (prec == KUQCD_DOUBLE)
             gauge_host_to_soa<HR, double>(stage.as<HR>(), i0, n, slot.soa.as<double2>(), Vh, scale, c.stream);
@@ -509,6 +509,39 @@ KuQCDStatus kq_solve_multishift_batched(int nrhs, void **x, void **b, KuQCDInver
             o = *opt;
         return solve_extended(slot, nrhs, x, b, *param, o, rep);
     });
+}
+
+void *kq_device_alloc(size_t bytes)
+{
+    try {
+        ensure_device();
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            set_error("hipMalloc failed");
+            return nullptr;
+        }
+        return p;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+void kq_device_free(void *p)
+{
+    if (p)
+        (void)hipFree(p);
+}
+int kq_memcpy(void *dst, const void *src, size_t bytes)
+{
+    try {
+        ensure_device();
+        KQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx().stream));
+        KQ_HIP_CHECK(hipStreamSynchronize(ctx().stream));
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
 }
 
 const char *kq_last_error(void)

@@ -20,7 +20,7 @@ static void apply_t(LF fat, LL lng, const Geom &g, int nrhs, const void *src, vo
     Vec<VR> sv{s.as<c2_t<VR>>(), g.Vh, n3}, dv{d.as<c2_t<VR>>(), g.Vh, n3};
     for (int r = 0; r < nrhs; ++r) {
         KQ_HIP_CHECK(hipMemcpyAsync(stage.as<void>(), static_cast<const HR *>(src) + r * nh, sizeof(HR) * nh,
-                                    hipMemcpyHostToDevice, c.stream));
+                                    hipMemcpyDefault, c.stream));
         vec_host_to_soa<HR, VR>(stage.as<HR>(), sv, r, c.stream);
     }
     DslashArgs<LF, LL, VR> a{};
@@ -55,7 +55,7 @@ static void apply_t(LF fat, LL lng, const Geom &g, int nrhs, const void *src, vo
     for (int r = 0; r < nrhs; ++r) {
         vec_soa_to_host<VR, HR>(dv, r, stage.as<HR>(), c.stream);
         KQ_HIP_CHECK(hipMemcpyAsync(static_cast<HR *>(dst) + r * nh, stage.as<void>(), sizeof(HR) * nh,
-                                    hipMemcpyDeviceToHost, c.stream));
+                                    hipMemcpyDefault, c.stream));
         KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
     }
 }

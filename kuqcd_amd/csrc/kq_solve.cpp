@@ -26,13 +26,13 @@ std::vector<double> reduce_host(const double *part_dev, uint32_t nblk, int n, hi
     return h;
 }
 
-// host vector (reference layout, HR) -> device SoA vector r of v
+// caller vector (reference layout, HR; host or device memory) -> device SoA vector r of v
 template <typename HR, typename VR> static void upload_vec(const void *h, Vec<VR> v, int r, DevBuf &stage, hipStream_t st)
 {
     const size_t n = (size_t)6 * v.Vh;
     if (stage.bytes() < sizeof(HR) * n)
         stage.alloc(sizeof(HR) * n);
-    KQ_HIP_CHECK(hipMemcpyAsync(stage.as<void>(), h, sizeof(HR) * n, hipMemcpyHostToDevice, st));
+    KQ_HIP_CHECK(hipMemcpyAsync(stage.as<void>(), h, sizeof(HR) * n, hipMemcpyDefault, st));
     vec_host_to_soa<HR, VR>(stage.as<HR>(), v, r, st);
     KQ_HIP_CHECK(hipStreamSynchronize(st));
 }
@@ -43,7 +43,7 @@ template <typename VR, typename HR> static void download_vec(Vec<VR> v, int r, v
     if (stage.bytes() < sizeof(HR) * n)
         stage.alloc(sizeof(HR) * n);
     vec_soa_to_host<VR, HR>(v, r, stage.as<HR>(), st);
-    KQ_HIP_CHECK(hipMemcpyAsync(h, stage.as<void>(), sizeof(HR) * n, hipMemcpyDeviceToHost, st));
+    KQ_HIP_CHECK(hipMemcpyAsync(h, stage.as<void>(), sizeof(HR) * n, hipMemcpyDefault, st));
     KQ_HIP_CHECK(hipStreamSynchronize(st));
 }
 

@@ -155,3 +155,19 @@ def test_status_codes_gpu(gpu):
     p = kq.InvertParam(shifts=np.array([0.001]), tol=1e-30, max_iter=0)
     assert kq.kuqcd_solve_multishift(x, b, p) == kq.KUQCD_SUCCESS
     assert p.iter == 1
+
+
+@pytest.mark.gpu
+def test_cpp_dropin_caller():
+    """An unmodified C++ caller of include/kuqcd.h (tests/cpp/dropin_test_solver.cpp)
+    runs the reference's known-answer case, reading the grid from stdin as
+    test/run.sh feeds test/input."""
+    import subprocess
+    from pathlib import Path
+    here = Path(__file__).resolve().parent / "cpp"
+    subprocess.run(["make", "-s", "-C", str(here)], check=True)
+    r = subprocess.run([str(here / "dropin_test_solver")], input="gridx=1\ngridy=1\ngridz=1\ngridt=1\n",
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "PASSED" in r.stdout
