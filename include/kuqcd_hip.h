@@ -69,6 +69,8 @@ typedef struct {
     double seconds_refine;
     int learned_used;                  /* 1 if the learned refinement was accepted */
     int learned_fallback;              /* 1 if it was rejected and plain CG refinement used */
+    double initial_rel_res[KQ_MAX_SHIFT]; /* FP64 true relative residual after the multishift phase */
+    double seconds_verify;             /* FP64 true-residual checks */
 } KqSolveReport;
 
 KQ_API void kq_default_options(KqSolveOptions *opt);

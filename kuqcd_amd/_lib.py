@@ -91,6 +91,8 @@ class KqSolveReport(C.Structure):
         ("seconds_refine", C.c_double),
         ("learned_used", C.c_int),
         ("learned_fallback", C.c_int),
+        ("initial_rel_res", C.c_double * KQ_MAX_SHIFT),
+        ("seconds_verify", C.c_double),
     ]
 
 

@@ -63,9 +63,10 @@ template <typename S, typename VR>
 void rel_fold(CgmState<S> *st, double *x_hi, VR *x_lo, int nshift, uint32_t Vh, const int *rel_list,
               const int *rel_n, int nsys, hipStream_t stream);
 
-// part[k][blk] = |b - ax_k|^2 for k < nk (b shared by all k; FP64)
-void resid2_partial(const double *b, const double *ax, int nk, uint32_t Vh, double *part, uint32_t nblk,
-                    hipStream_t stream);
+// part[k][blk] = |b - ax_k|^2 for k = list[0..nk) (or k < nk when list is null);
+// b shared by all k, FP64
+void resid2_partial(const double *b, const double *ax, int nk, const int *list, uint32_t Vh, double *part,
+                    uint32_t nblk, hipStream_t stream);
 
 // ---- generic BLAS-1 on colour vectors with deterministic partial sums
 // part[r][blk] = sum over the block's sites of Re<x_r, y_r>

@@ -357,11 +357,11 @@ void rel_fold(CgmState<S> *st, double *x_hi, VR *x_lo, int nshift, uint32_t Vh, 
 }
 
 // ------------------------------------------------------------ true residuals
-__global__ void __launch_bounds__(kBlasNT) k_resid2(const double2 *b, const double2 *ax, uint32_t Vh, double *part,
-                                                    uint32_t nblk)
+__global__ void __launch_bounds__(kBlasNT) k_resid2(const double2 *b, const double2 *ax, const int *list, uint32_t Vh,
+                                                    double *part, uint32_t nblk)
 {
     __shared__ double lds[kBlasNT / kWave];
-    const int k = blockIdx.y;
+    const int k = list ? list[blockIdx.y] : (int)blockIdx.y;
     const uint32_t i = blockIdx.x * kBlasNT + threadIdx.x;
     double acc = 0.0;
     if (i < Vh) {
@@ -378,11 +378,13 @@ __global__ void __launch_bounds__(kBlasNT) k_resid2(const double2 *b, const doub
         part[(size_t)k * nblk + blockIdx.x] = tot;
 }
 
-void resid2_partial(const double *b, const double *ax, int nk, uint32_t Vh, double *part, uint32_t nblk,
-                    hipStream_t stream)
+void resid2_partial(const double *b, const double *ax, int nk, const int *list, uint32_t Vh, double *part,
+                    uint32_t nblk, hipStream_t stream)
 {
+    if (nk <= 0)
+        return;
     hipLaunchKernelGGL(k_resid2, dim3(nblk, nk), dim3(kBlasNT), 0, stream, reinterpret_cast<const double2 *>(b),
-                       reinterpret_cast<const double2 *>(ax), Vh, part, nblk);
+                       reinterpret_cast<const double2 *>(ax), list, Vh, part, nblk);
     KQ_HIP_CHECK(hipGetLastError());
 }
 

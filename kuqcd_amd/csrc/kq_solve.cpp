@@ -3,8 +3,10 @@
 // batched refinement.
 #include "kq_solve.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -97,40 +99,64 @@ KuQCDStatus solve_reference(int slot, void **x, void *b, KuQCDInvertParam &p)
 }
 
 // ---------------------------------------------------------------- FP64 verification
-// res2[k] = |b - A_k x_k|^2 for the nk vectors x[k] (FP64 SoA, stride 3Vh),
-// A_k = sigma_k - D D with absolute shifts sigma_k.
+// res2[k] = |b - A_k x_k|^2 for the k in `which` (all nk when empty), x[k] FP64
+// SoA with stride 3Vh, A_k = sigma_k - D D with absolute shifts sigma_k. The
+// FP64 hopping term runs batched over the listed vectors (links read once per
+// pair of vectors).
 static std::vector<double> true_res2(int slot, int parity, const double *b, const double *x, int nk,
-                                     const std::vector<double> &sigma_abs)
+                                     const std::vector<double> &sigma_abs, std::vector<int> which = {})
 {
     Context &c = ctx();
     const Geom g = slot_geom(slot);
     const size_t n3 = (size_t)3 * g.Vh;
+    if (which.empty())
+        for (int k = 0; k < nk; ++k)
+            which.push_back(k);
+    const int nl = (int)which.size();
     DevBuf tmp(sizeof(double2) * nk * n3), ax(sizeof(double2) * nk * n3), sig(sizeof(double) * nk);
-    const uint32_t nblk_d = dslash_nblk(g.Vh, 256);
+    DevBuf list(sizeof(int) * (nl + 1));
+    const int nt = 256, nr = nl >= 2 ? 2 : 1;
+    const uint32_t nblk_d = dslash_nblk(g.Vh, nt);
     DevBuf part(sizeof(double) * nk * std::max(nblk_d, blas_nblk(g.Vh)));
+    std::vector<int> hl(which);
+    hl.push_back(nl);
     KQ_HIP_CHECK(hipMemcpyAsync(sig.as<void>(), sigma_abs.data(), sizeof(double) * nk, hipMemcpyHostToDevice,
                                 c.stream));
+    KQ_HIP_CHECK(hipMemcpyAsync(list.as<void>(), hl.data(), sizeof(int) * (nl + 1), hipMemcpyHostToDevice, c.stream));
     DslashArgs<LinkD, LinkD, double> a{};
     a.g = g;
     a.fat = link_d(c.fat[slot]);
     a.lng = link_d(c.lng[slot]);
-    a.nrhs = nk;
+    a.nrhs = nl;
+    a.rlist = list.as<int>();
+    a.nact = list.as<int>() + nl;
     Vec<double> xv{reinterpret_cast<double2 *>(const_cast<double *>(x)), g.Vh, n3};
     Vec<double> tv{tmp.as<double2>(), g.Vh, n3}, av{ax.as<double2>(), g.Vh, n3};
     a.dst_parity = 1 - parity;
     a.src = xv;
     a.dst = tv;
-    launch_dslash(a, EPI_STORE, 1, 256, c.stream);
+    launch_dslash(a, EPI_STORE, nr, nt, c.stream);
     a.dst_parity = parity;
     a.src = tv;
     a.dst = av;
     a.aux = xv;
     a.sigma = sig.as<double>();
     a.partial = part.as<double>();
-    launch_dslash(a, EPI_AXMY_DOT, 1, 256, c.stream);
+    launch_dslash(a, EPI_AXMY_DOT, nr, nt, c.stream);
     const uint32_t nb = blas_nblk(g.Vh);
-    resid2_partial(b, ax.as<double>(), nk, g.Vh, part.as<double>(), nb, c.stream);
-    return reduce_host(part.as<double>(), nb, nk, c.stream);
+    resid2_partial(b, ax.as<double>(), nl, list.as<int>(), g.Vh, part.as<double>(), nb, c.stream);
+    const std::vector<double> all = reduce_host(part.as<double>(), nb, nk, c.stream);
+    std::vector<double> out(nk, 0.0);
+    for (int k : which)
+        out[k] = all[k];
+    KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+    return out;
+}
+
+static bool debug_on()
+{
+    static const bool on = getenv("KQ_DEBUG") != nullptr;
+    return on;
 }
 
 static std::vector<double> abs_shifts(const KuQCDInvertParam &p, int r)
@@ -195,17 +221,37 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
 
     // per-shift FP64 verification and batched refinement
     res2.assign((size_t)P.nrhs * P.ns, 0.0);
-    auto verify = [&]() {
+    double t_verify = 0.0;
+    // verify the listed (rhs, shift) pairs; all of them when the list is empty
+    auto verify = [&](const std::vector<std::pair<int, int>> &pairs) {
+        auto tv0 = clk::now();
         for (int r = 0; r < P.nrhs; ++r) {
+            std::vector<int> ks;
+            for (const auto &pr : pairs)
+                if (pr.first == r)
+                    ks.push_back(pr.second);
+            if (!pairs.empty() && ks.empty())
+                continue;
             const auto v = true_res2(P.slot, P.parity, P.b->as<double>() + (size_t)r * 2 * n3,
-                                     x_out.as<double>() + (size_t)r * P.ns * 2 * n3, P.ns, abs_shifts(*P.p, r));
+                                     x_out.as<double>() + (size_t)r * P.ns * 2 * n3, P.ns, abs_shifts(*P.p, r), ks);
             for (int k = 0; k < P.ns; ++k)
-                res2[(size_t)r * P.ns + k] = v[k];
+                if (pairs.empty() || std::find(ks.begin(), ks.end(), k) != ks.end())
+                    res2[(size_t)r * P.ns + k] = v[k];
         }
+        t_verify += secs(tv0, clk::now());
     };
-    verify();
-    if (!P.o->refine)
+    verify({});
+    for (int r = 0; r < P.nrhs; ++r)
+        for (int k = 0; k < P.ns; ++k)
+            rep[r].initial_rel_res[k] = std::sqrt(res2[(size_t)r * P.ns + k] / P.b2[r]);
+    if (debug_on())
+        for (int k = 0; k < P.ns; ++k)
+            fprintf(stderr, "[kq] shift %d initial true rel res %.3e\n", k, rep[0].initial_rel_res[k]);
+    if (!P.o->refine) {
+        for (int r = 0; r < P.nrhs; ++r)
+            rep[r].seconds_verify = t_verify;
         return;
+    }
     for (int round = 0; round < 3; ++round) {
         std::vector<std::pair<int, int>> need;
         for (int r = 0; r < P.nrhs; ++r)
@@ -244,12 +290,13 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
             }
         }
         KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
-        verify();
+        verify(need);
     }
     auto t2 = clk::now();
-    if (rep)
-        for (int r = 0; r < P.nrhs; ++r)
-            rep[r].seconds_refine = secs(t1, t2);
+    for (int r = 0; r < P.nrhs; ++r) {
+        rep[r].seconds_refine = secs(t1, t2) - t_verify;
+        rep[r].seconds_verify = t_verify;
+    }
 }
 
 static void run_fp64(Problem &P, DevBuf &x_out, KqSolveReport *rep, std::vector<double> &res2)
@@ -282,6 +329,7 @@ static void run_fp64(Problem &P, DevBuf &x_out, KqSolveReport *rep, std::vector<
             rep[r].seconds_multishift = secs(t0, t1);
         }
     res2.assign((size_t)P.nrhs * P.ns, 0.0);
+    auto tv0 = clk::now();
     if (P.o->refine)
         for (int r = 0; r < P.nrhs; ++r) {
             const auto v = true_res2(P.slot, P.parity, P.b->as<double>() + (size_t)r * 2 * n3,
@@ -289,6 +337,11 @@ static void run_fp64(Problem &P, DevBuf &x_out, KqSolveReport *rep, std::vector<
             for (int k = 0; k < P.ns; ++k)
                 res2[(size_t)r * P.ns + k] = v[k];
         }
+    for (int r = 0; r < P.nrhs; ++r) {
+        rep[r].seconds_verify = secs(tv0, clk::now());
+        for (int k = 0; k < P.ns; ++k)
+            rep[r].initial_rel_res[k] = std::sqrt(res2[(size_t)r * P.ns + k] / P.b2[r]);
+    }
 }
 
 KuQCDStatus solve_extended(int slot, int nrhs, void **x, void **b, KuQCDInvertParam &p, const KqSolveOptions &o,

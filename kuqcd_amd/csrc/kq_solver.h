@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -226,8 +228,12 @@ public:
             KQ_HIP_CHECK(hipStreamSynchronize(st_));
             bool live = false;
             std::vector<int> rel;
+            static const bool dbg = getenv("KQ_DEBUG") != nullptr;
             for (int r = 0; r < nsys_; ++r) {
                 const CgmState<S> &c = state_h_[r];
+                if (dbg)
+                    fprintf(stderr, "[kq] sys %d iter %d rr %.4e r2max %.4e maxTerm %d stop %d B0 %.4e a0 %.4e\n", r,
+                            c.iter, (double)c.rr, c.r2max, c.maxTerm, c.stop, (double)c.B[0], (double)c.a[0]);
                 if (!c.stop)
                     live = true;
                 if (reliable_) {
