@@ -55,6 +55,9 @@ typedef struct {
     int nr;                /* right-hand sides per thread of the batched hopping term (0: autotune) */
     int nt;                /* threads per block of the hopping term (0: autotune) */
     int verbose;           /* 0 silent, 1 iteration summary (reference print), 2 details */
+    double fork_tol;       /* mixed modes: a shift k >= 1 leaves the multishift and continues as its own
+                              FP64-reliable CG once its iterated relative residual is below fork_tol
+                              (0: automatic, 1e-5 for FP32 and 1e-3 for FP16 links; < 0: never) */
 } KqSolveOptions;
 
 #define KQ_MAX_SHIFT 32

@@ -76,6 +76,7 @@ class KqSolveOptions(C.Structure):
         ("nr", C.c_int),
         ("nt", C.c_int),
         ("verbose", C.c_int),
+        ("fork_tol", C.c_double),
     ]
 
 

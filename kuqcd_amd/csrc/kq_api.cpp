@@ -463,6 +463,7 @@ void kq_default_options(KqSolveOptions *opt)
     opt->nr = 0;
     opt->nt = 0;
     opt->verbose = 1;
+    opt->fork_tol = 0.0;
 }
 
 KuQCDStatus kq_solve_multishift_ex(void **x, void *b, KuQCDInvertParam *param, const KqSolveOptions *opt,

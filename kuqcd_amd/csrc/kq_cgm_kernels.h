@@ -1,10 +1,15 @@
 // Device-resident multishift CG state and the kernels of one CGM iteration.
 //
-// One CgmState per right-hand side holds the Jegerlehner recurrence scalars of
-// CGM::Invert (inverter_cgm.cpp:20-78). Every kernel of an iteration reads the
-// state from device memory, so an iteration needs no host round trip and can
-// be captured once in a hipGraph and replayed; a right-hand side that has
-// converged (or hit max_iter) sets `stop`, after which the kernels skip it.
+// One CgmState per system (a right-hand side with its list of shifts) holds the
+// Jegerlehner recurrence scalars of CGM::Invert (inverter_cgm.cpp:20-78).
+// Every kernel of an iteration reads the states from device memory, so an
+// iteration needs no host round trip and is captured once in a hipGraph and
+// replayed; a system that has converged (or hit max_iter) sets `stop`, and one
+// that needs a reliable update may set `pause`; the kernels skip both.
+//
+// A system's search directions p_k and solutions x_k live at element offset
+// off[sys] + k*3*Vh of their arrays (complex units), so systems with different
+// numbers of shifts share one pool.
 #pragma once
 
 #include "kq_fields.h"
@@ -17,22 +22,24 @@ template <typename S> struct CgmState {
     S rr;   // |r|^2 of the current residual
     S Bm1;  // beta_0 of the previous iteration
     S a[kMaxShift], B[kMaxShift], Z[kMaxShift], Zm1[kMaxShift];
-    S shift[kMaxShift];  // reference convention: shift[0] = sigma_0, shift[k>=1] relative to it
-    double tol;          // absolute squared-residual target (inverter_cgm.cpp:72-73)
-    double r2max;        // |r|^2 at the last reliable update (mixed precision only)
-    int nshift, maxTerm, maxTermUpd, iter, maxIter, stop, run, maxTermFold;
+    S shift[kMaxShift];       // reference convention: shift[0] = sigma_0, shift[k>=1] relative to it
+    double tolk[kMaxShift];   // absolute squared-residual target per shift (inverter_cgm.cpp:72-73)
+    double r2max;             // |r|^2 at the last reliable update (mixed precision)
+    double pause2;            // pause when |r|^2 < pause2 * r2max (0: never)
+    int nshift, maxTerm, maxTermUpd, iter, maxIter, stop, run, maxTermFold, pause, pad;
 };
 
 struct CgmLaunch {
     uint32_t Vh;
     uint32_t nblk;      // blocks of the BLAS site loops (= stride of part_b)
     uint32_t nblk_dsl;  // blocks of the hopping-term launch (= stride of part_a)
-    int nrhs;        // right-hand sides in the batch
-    int *rlist;      // compact list of live right-hand sides (written by k_cgm_begin)
-    int *nact;       // its length
-    int *any;        // 1 while any right-hand side is live
-    double *part_a;  // [nrhs][nblk_dsl] partial sums of <p, A p>
-    double *part_b;  // [nrhs][nblk] partial sums of |r|^2
+    int nsys;           // system slots
+    int *rlist;         // compact list of live systems (written by k_cgm_begin)
+    int *nact;          // its length
+    int *any;           // 1 while any system is live
+    double *part_a;     // [nsys][nblk_dsl] partial sums of <p, A p>
+    double *part_b;     // [nsys][nblk] partial sums of |r|^2
+    const size_t *off;  // [nsys] element offset of each system's p/x block
 };
 
 template <typename S> void cgm_begin(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream);
@@ -42,26 +49,32 @@ template <typename S> void cgm_scalars2(CgmState<S> *st, const CgmLaunch &L, hip
 template <typename S, typename VR>
 void cgm_axpy_norm(CgmState<S> *st, Vec<VR> r, Vec<VR> s, const CgmLaunch &L, hipStream_t stream);
 // x_k -= beta_k p_k ; p_k = zeta_k r + alpha_k p_k      (inverter_cgm.cpp:61-69)
-// x and p: right-hand side r, shift k at base + (r*nshift + k)*3*Vh.
 template <typename S, typename VR, typename XR>
-void cgm_xp_update(CgmState<S> *st, XR *x, VR *p, Vec<VR> r, int nshift, const CgmLaunch &L, hipStream_t stream);
-// state init: rr = |b|^2 from part_a, scalars as inverter_cgm.cpp:30-39
-template <typename S>
-void cgm_init_state(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream);
+void cgm_xp_update(CgmState<S> *st, XR *x, VR *p, Vec<VR> r, const CgmLaunch &L, hipStream_t stream);
 
-// ---- reliable updates (mixed precision): systems listed in rel_list[0..*rel_n)
-// r_hi = b - Ax (Ax = A_0 x_hi computed by the FP64 hopping term), r_lo = (VR) r_hi,
-// part[r][blk] = |r_hi|^2; rr = r2max = |r_hi|^2 is then set by rel_set_rr.
+// ---- start of listed systems (list[0..*n)): rr = r2max = sum of part_b,
+// scalars as inverter_cgm.cpp:30-39, stop = pause = 0 (stop = 1 if nshift <= 0)
+template <typename S>
+void cgm_init_list(CgmState<S> *st, const int *list, const int *n, int nsys, const double *part, uint32_t nblk,
+                   hipStream_t stream);
+
+// ---- reliable updates (mixed precision), listed systems
+// r_hi = b - Ax (Ax = A_0 x_hi by the FP64 hopping term), r_lo = (VR) r_hi,
+// part[sys][blk] = |r_hi|^2; rel_set_rr then sets rr = r2max = |r_hi|^2.
 template <typename VR>
-void rel_residual(const double *b, const double *ax, Vec<VR> r_lo, const int *rel_list, const int *rel_n, int nsys,
+void rel_residual(const double *b, const double *ax, Vec<VR> r_lo, const int *list, const int *n, int nsys,
                   double *part, uint32_t nblk, hipStream_t stream);
 template <typename S>
-void rel_set_rr(CgmState<S> *st, const int *rel_list, const int *rel_n, int nsys, const double *part, uint32_t nblk,
+void rel_set_rr(CgmState<S> *st, const int *list, const int *n, int nsys, const double *part, uint32_t nblk,
                 hipStream_t stream);
-// x_hi[r][k] += x_lo[r][k]; x_lo[r][k] = 0 for k < st[r].maxTermFold
+// x_hi[k] += x_lo[k]; x_lo[k] = 0 for k < st[sys].maxTermFold
 template <typename S, typename VR>
-void rel_fold(CgmState<S> *st, double *x_hi, VR *x_lo, int nshift, uint32_t Vh, const int *rel_list,
-              const int *rel_n, int nsys, hipStream_t stream);
+void rel_fold(CgmState<S> *st, double *x_hi, VR *x_lo, const size_t *off, uint32_t Vh, const int *list, const int *n,
+              int nsys, hipStream_t stream);
+// p_k = r for k < st[sys].nshift (start of a system)
+template <typename S, typename VR>
+void copy_r_to_p(const CgmState<S> *st, Vec<VR> r, VR *p, const size_t *off, const int *list, const int *n, int nsys,
+                 hipStream_t stream);
 
 // part[k][blk] = |b - ax_k|^2 for k = list[0..nk) (or k < nk when list is null);
 // b shared by all k, FP64
@@ -69,9 +82,10 @@ void resid2_partial(const double *b, const double *ax, int nk, const int *list, 
                     uint32_t nblk, hipStream_t stream);
 
 // ---- generic BLAS-1 on colour vectors with deterministic partial sums
-// part[r][blk] = sum over the block's sites of Re<x_r, y_r>
+// part[r][blk] = sum over the block's sites of Re<x_r, y_r> for r = list[..] (all r < nrhs if null)
 template <typename VR>
-void vec_dot_partial(Vec<VR> x, Vec<VR> y, int nrhs, double *part, uint32_t nblk, hipStream_t stream);
+void vec_dot_partial(Vec<VR> x, Vec<VR> y, int nrhs, double *part, uint32_t nblk, hipStream_t stream,
+                     const int *list = nullptr, const int *n = nullptr);
 // out[r] = sum_blk part[r][blk] (fixed order)
 void reduce_partials(const double *part, uint32_t nblk, int nrhs, double *out, hipStream_t stream);
 

@@ -1,5 +1,6 @@
 // Kernels of one multishift CG iteration (CGM::Invert, inverter_cgm.cpp:42-78)
-// with the recurrence scalars kept in device memory.
+// with the recurrence scalars kept in device memory, plus the start-up and
+// reliable-update kernels of the mixed-precision solver.
 //
 // Per iteration:  begin -> D (p0 -> tmp) -> D + axmy + <p0,s> -> scalars1 ->
 //                 r += b0 s, |r|^2 -> scalars2 -> x/p update of every live shift
@@ -28,13 +29,13 @@ __device__ inline double block_reduce_fixed(const double *part, uint32_t n)
 }
 
 // ------------------------------------------------------------ begin
-template <typename S> __global__ void k_cgm_begin(CgmState<S> *st, int nrhs, int *rlist, int *nact, int *any)
+template <typename S> __global__ void k_cgm_begin(CgmState<S> *st, int nsys, int *rlist, int *nact, int *any)
 {
     if (threadIdx.x != 0)
         return;
     int n = 0;
-    for (int r = 0; r < nrhs; ++r) {
-        const int live = st[r].stop == 0;
+    for (int r = 0; r < nsys; ++r) {
+        const int live = (st[r].stop == 0) && (st[r].pause == 0);
         st[r].run = live;
         if (live)
             rlist[n++] = r;
@@ -45,7 +46,7 @@ template <typename S> __global__ void k_cgm_begin(CgmState<S> *st, int nrhs, int
 
 template <typename S> void cgm_begin(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_cgm_begin<S>, dim3(1), dim3(64), 0, stream, st, L.nrhs, L.rlist, L.nact, L.any);
+    hipLaunchKernelGGL(k_cgm_begin<S>, dim3(1), dim3(64), 0, stream, st, L.nsys, L.rlist, L.nact, L.any);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
@@ -85,14 +86,15 @@ __global__ void __launch_bounds__(256) k_cgm_scalars1(CgmState<S> *st, const int
 
 template <typename S> void cgm_scalars1(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_cgm_scalars1<S>, dim3(L.nrhs), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_a,
+    hipLaunchKernelGGL(k_cgm_scalars1<S>, dim3(L.nsys), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_a,
                        L.nblk_dsl);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
 // ------------------------------------------------------------ scalars 2
 // |r|^2 new, alpha recurrences, drop converged shifts from the end,
-// iteration count and stop flag (inverter_cgm.cpp:58-78).
+// iteration count and stop flag (inverter_cgm.cpp:58-78); pause flag for the
+// reliable-update schedule.
 template <typename S>
 __global__ void __launch_bounds__(256) k_cgm_scalars2(CgmState<S> *st, const int *rlist, const int *nact,
                                                       const double *part, uint32_t nblk)
@@ -112,20 +114,23 @@ __global__ void __launch_bounds__(256) k_cgm_scalars2(CgmState<S> *st, const int
     for (int j = 1; j < c.maxTerm; ++j)
         c.a[j] = c.a[0] * c.Z[j] * c.B[j] / (c.Zm1[j] * c.B[0]);
     c.maxTermUpd = c.maxTerm;
-    S lambda;
-    do {
-        lambda = c.Z[c.maxTerm - 1] * c.Z[c.maxTerm - 1] * lambda2;
-        if ((double)lambda < c.tol)
-            c.maxTerm--;
-    } while (c.maxTerm > 0 && (double)lambda < c.tol);
+    // drop converged shifts from the end of the list (inverter_cgm.cpp:71-76)
+    while (c.maxTerm > 0) {
+        const S lambda = c.Z[c.maxTerm - 1] * c.Z[c.maxTerm - 1] * lambda2;
+        if (!((double)lambda < c.tolk[c.maxTerm - 1]))
+            break;
+        c.maxTerm--;
+    }
     c.iter += 1;
     if (c.maxTerm <= 0 || c.iter >= c.maxIter)
         c.stop = 1;
+    if (c.pause2 > 0.0 && l2_d < c.pause2 * c.r2max)
+        c.pause = 1;
 }
 
 template <typename S> void cgm_scalars2(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_cgm_scalars2<S>, dim3(L.nrhs), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_b,
+    hipLaunchKernelGGL(k_cgm_scalars2<S>, dim3(L.nsys), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_b,
                        L.nblk);
     KQ_HIP_CHECK(hipGetLastError());
 }
@@ -165,15 +170,15 @@ __global__ void __launch_bounds__(kBlasNT) k_axpy_norm(const CgmState<S> *st, Ve
 template <typename S, typename VR>
 void cgm_axpy_norm(CgmState<S> *st, Vec<VR> r, Vec<VR> s, const CgmLaunch &L, hipStream_t stream)
 {
-    hipLaunchKernelGGL((k_axpy_norm<S, VR>), dim3(L.nblk, L.nrhs), dim3(kBlasNT), 0, stream, st, r, s, L.rlist,
+    hipLaunchKernelGGL((k_axpy_norm<S, VR>), dim3(L.nblk, L.nsys), dim3(kBlasNT), 0, stream, st, r, s, L.rlist,
                        L.nact, L.part_b, L.nblk);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
 // ------------------------------------------------------------ x/p update
 template <typename S, typename VR, typename XR>
-__global__ void __launch_bounds__(kBlasNT) k_xp_update(const CgmState<S> *st, XR *x, VR *p, Vec<VR> r, int nshift,
-                                                       const int *rlist, const int *nact)
+__global__ void __launch_bounds__(kBlasNT) k_xp_update(const CgmState<S> *st, XR *x, VR *p, Vec<VR> r,
+                                                       const size_t *off, const int *rlist, const int *nact)
 {
     using V2 = c2_t<VR>;
     using X2 = c2_t<XR>;
@@ -187,10 +192,11 @@ __global__ void __launch_bounds__(kBlasNT) k_xp_update(const CgmState<S> *st, XR
         return;
     const CgmState<S> &c = st[rh];
     const int m = c.maxTermUpd;
+    const size_t o = off[rh];
     V2 rv[3];
     r.load(rh, i, rv);
     for (int k = 0; k < m; ++k) {
-        const size_t base = ((size_t)rh * nshift + k) * 3 * Vh + i;
+        const size_t base = o + (size_t)k * 3 * Vh + i;
         const XR bk = (XR)c.B[k];
         const VR zk = (VR)c.Z[k], ak = (VR)c.a[k];
         V2 *pp = reinterpret_cast<V2 *>(p) + base;
@@ -211,17 +217,22 @@ __global__ void __launch_bounds__(kBlasNT) k_xp_update(const CgmState<S> *st, XR
 }
 
 template <typename S, typename VR, typename XR>
-void cgm_xp_update(CgmState<S> *st, XR *x, VR *p, Vec<VR> r, int nshift, const CgmLaunch &L, hipStream_t stream)
+void cgm_xp_update(CgmState<S> *st, XR *x, VR *p, Vec<VR> r, const CgmLaunch &L, hipStream_t stream)
 {
-    hipLaunchKernelGGL((k_xp_update<S, VR, XR>), dim3(L.nblk, L.nrhs), dim3(kBlasNT), 0, stream, st,
-                       reinterpret_cast<XR *>(x), reinterpret_cast<VR *>(p), r, nshift, L.rlist, L.nact);
+    hipLaunchKernelGGL((k_xp_update<S, VR, XR>), dim3(L.nblk, L.nsys), dim3(kBlasNT), 0, stream, st, x, p, r, L.off,
+                       L.rlist, L.nact);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
-// ------------------------------------------------------------ init
-template <typename S> __global__ void __launch_bounds__(256) k_cgm_init(CgmState<S> *st, const double *part, uint32_t nblk)
+// ------------------------------------------------------------ start of systems
+template <typename S>
+__global__ void __launch_bounds__(256) k_cgm_init_list(CgmState<S> *st, const int *list, const int *n,
+                                                       const double *part, uint32_t nblk)
 {
-    const int r = blockIdx.x;
+    const int slot = blockIdx.x;
+    if (*n <= slot)
+        return;
+    const int r = list[slot];
     const double rr = block_reduce_fixed(part + (size_t)r * nblk, nblk);
     if (threadIdx.x != 0)
         return;
@@ -239,29 +250,64 @@ template <typename S> __global__ void __launch_bounds__(256) k_cgm_init(CgmState
     c.maxTermFold = c.nshift;
     c.r2max = rr;
     c.iter = 0;
+    c.pause = 0;
     // the reference's do-while runs at least one iteration even for max_iter <= 0
     c.stop = (c.nshift <= 0) ? 1 : 0;
     c.run = 0;
 }
 
-template <typename S> void cgm_init_state(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
+template <typename S>
+void cgm_init_list(CgmState<S> *st, const int *list, const int *n, int nsys, const double *part, uint32_t nblk,
+                   hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_cgm_init<S>, dim3(L.nrhs), dim3(256), 0, stream, st, L.part_b, L.nblk);
+    hipLaunchKernelGGL(k_cgm_init_list<S>, dim3(nsys), dim3(256), 0, stream, st, list, n, part, nblk);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+template <typename S, typename VR>
+__global__ void __launch_bounds__(kBlasNT) k_copy_r_to_p(const CgmState<S> *st, Vec<VR> r, VR *p, const size_t *off,
+                                                         const int *list, const int *n)
+{
+    using V2 = c2_t<VR>;
+    const int slot = blockIdx.y;
+    if (*n <= slot)
+        return;
+    const int rh = list[slot];
+    const uint32_t i = blockIdx.x * kBlasNT + threadIdx.x;
+    const uint32_t Vh = r.Vh;
+    if (i >= Vh)
+        return;
+    V2 rv[3];
+    r.load(rh, i, rv);
+    const int m = st[rh].nshift;
+    for (int k = 0; k < m; ++k) {
+        V2 *pp = reinterpret_cast<V2 *>(p) + off[rh] + (size_t)k * 3 * Vh + i;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            pp[(size_t)c * Vh] = rv[c];
+    }
+}
+
+template <typename S, typename VR>
+void copy_r_to_p(const CgmState<S> *st, Vec<VR> r, VR *p, const size_t *off, const int *list, const int *n, int nsys,
+                 hipStream_t stream)
+{
+    hipLaunchKernelGGL((k_copy_r_to_p<S, VR>), dim3(blas_nblk(r.Vh), nsys), dim3(kBlasNT), 0, stream, st, r, p, off,
+                       list, n);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
 // ------------------------------------------------------------ reliable updates
 template <typename VR>
 __global__ void __launch_bounds__(kBlasNT) k_rel_residual(const double *b, const double *ax, Vec<VR> r_lo,
-                                                          const int *rel_list, const int *rel_n, double *part,
-                                                          uint32_t nblk)
+                                                          const int *list, const int *n, double *part, uint32_t nblk)
 {
     using V2 = c2_t<VR>;
     __shared__ double lds[kBlasNT / kWave];
     const int slot = blockIdx.y;
-    if (*rel_n <= slot)
+    if (*n <= slot)
         return;
-    const int rh = rel_list[slot];
+    const int rh = list[slot];
     const uint32_t Vh = r_lo.Vh;
     const uint32_t i = blockIdx.x * kBlasNT + threadIdx.x;
     double acc = 0.0;
@@ -285,22 +331,22 @@ __global__ void __launch_bounds__(kBlasNT) k_rel_residual(const double *b, const
 }
 
 template <typename VR>
-void rel_residual(const double *b, const double *ax, Vec<VR> r_lo, const int *rel_list, const int *rel_n, int nsys,
+void rel_residual(const double *b, const double *ax, Vec<VR> r_lo, const int *list, const int *n, int nsys,
                   double *part, uint32_t nblk, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_rel_residual<VR>, dim3(nblk, nsys), dim3(kBlasNT), 0, stream, b, ax, r_lo, rel_list, rel_n,
-                       part, nblk);
+    hipLaunchKernelGGL(k_rel_residual<VR>, dim3(nblk, nsys), dim3(kBlasNT), 0, stream, b, ax, r_lo, list, n, part,
+                       nblk);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
 template <typename S>
-__global__ void __launch_bounds__(256) k_rel_set_rr(CgmState<S> *st, const int *rel_list, const int *rel_n,
+__global__ void __launch_bounds__(256) k_rel_set_rr(CgmState<S> *st, const int *list, const int *n,
                                                     const double *part, uint32_t nblk)
 {
     const int slot = blockIdx.x;
-    if (*rel_n <= slot)
+    if (*n <= slot)
         return;
-    const int rh = rel_list[slot];
+    const int rh = list[slot];
     const double rr = block_reduce_fixed(part + (size_t)rh * nblk, nblk);
     if (threadIdx.x == 0) {
         st[rh].rr = (S)rr;
@@ -309,29 +355,29 @@ __global__ void __launch_bounds__(256) k_rel_set_rr(CgmState<S> *st, const int *
 }
 
 template <typename S>
-void rel_set_rr(CgmState<S> *st, const int *rel_list, const int *rel_n, int nsys, const double *part, uint32_t nblk,
+void rel_set_rr(CgmState<S> *st, const int *list, const int *n, int nsys, const double *part, uint32_t nblk,
                 hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_rel_set_rr<S>, dim3(nsys), dim3(256), 0, stream, st, rel_list, rel_n, part, nblk);
+    hipLaunchKernelGGL(k_rel_set_rr<S>, dim3(nsys), dim3(256), 0, stream, st, list, n, part, nblk);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
 template <typename S, typename VR>
-__global__ void __launch_bounds__(kBlasNT) k_rel_fold(const CgmState<S> *st, double *x_hi, VR *x_lo, int nshift,
-                                                      uint32_t Vh, const int *rel_list, const int *rel_n)
+__global__ void __launch_bounds__(kBlasNT) k_rel_fold(const CgmState<S> *st, double *x_hi, VR *x_lo,
+                                                      const size_t *off, uint32_t Vh, const int *list, const int *n)
 {
     using V2 = c2_t<VR>;
     const int slot = blockIdx.y;
-    if (*rel_n <= slot)
+    if (*n <= slot)
         return;
-    const int rh = rel_list[slot];
+    const int rh = list[slot];
     const size_t n3 = (size_t)3 * Vh;
     const size_t i = (size_t)blockIdx.x * kBlasNT + threadIdx.x;
     if (i >= n3)
         return;
     const int m = st[rh].maxTermFold;
     for (int k = 0; k < m; ++k) {
-        const size_t e = ((size_t)rh * nshift + k) * n3 + i;
+        const size_t e = off[rh] + (size_t)k * n3 + i;
         V2 *xl = reinterpret_cast<V2 *>(x_lo) + e;
         double2 *xh = reinterpret_cast<double2 *>(x_hi) + e;
         const V2 l = *xl;
@@ -347,12 +393,12 @@ __global__ void __launch_bounds__(kBlasNT) k_rel_fold(const CgmState<S> *st, dou
 }
 
 template <typename S, typename VR>
-void rel_fold(CgmState<S> *st, double *x_hi, VR *x_lo, int nshift, uint32_t Vh, const int *rel_list,
-              const int *rel_n, int nsys, hipStream_t stream)
+void rel_fold(CgmState<S> *st, double *x_hi, VR *x_lo, const size_t *off, uint32_t Vh, const int *list, const int *n,
+              int nsys, hipStream_t stream)
 {
     const size_t n3 = (size_t)3 * Vh;
     hipLaunchKernelGGL((k_rel_fold<S, VR>), dim3((unsigned)((n3 + kBlasNT - 1) / kBlasNT), nsys), dim3(kBlasNT), 0,
-                       stream, st, x_hi, x_lo, nshift, Vh, rel_list, rel_n);
+                       stream, st, x_hi, x_lo, off, Vh, list, n);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
@@ -390,11 +436,14 @@ void resid2_partial(const double *b, const double *ax, int nk, const int *list, 
 
 // ------------------------------------------------------------ generic dot
 template <typename VR>
-__global__ void __launch_bounds__(kBlasNT) k_dot_partial(Vec<VR> x, Vec<VR> y, double *part, uint32_t nblk)
+__global__ void __launch_bounds__(kBlasNT) k_dot_partial(Vec<VR> x, Vec<VR> y, double *part, uint32_t nblk,
+                                                         const int *list, const int *n)
 {
     using V2 = c2_t<VR>;
     __shared__ double lds[kBlasNT / kWave];
-    const int rh = blockIdx.y;
+    if (n != nullptr && *n <= (int)blockIdx.y)
+        return;
+    const int rh = list ? list[blockIdx.y] : (int)blockIdx.y;
     const uint32_t i = blockIdx.x * kBlasNT + threadIdx.x;
     double acc = 0.0;
     if (i < x.Vh) {
@@ -411,9 +460,10 @@ __global__ void __launch_bounds__(kBlasNT) k_dot_partial(Vec<VR> x, Vec<VR> y, d
 }
 
 template <typename VR>
-void vec_dot_partial(Vec<VR> x, Vec<VR> y, int nrhs, double *part, uint32_t nblk, hipStream_t stream)
+void vec_dot_partial(Vec<VR> x, Vec<VR> y, int nrhs, double *part, uint32_t nblk, hipStream_t stream, const int *list,
+                     const int *n)
 {
-    hipLaunchKernelGGL(k_dot_partial<VR>, dim3(nblk, nrhs), dim3(kBlasNT), 0, stream, x, y, part, nblk);
+    hipLaunchKernelGGL(k_dot_partial<VR>, dim3(nblk, nrhs), dim3(kBlasNT), 0, stream, x, y, part, nblk, list, n);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
@@ -432,40 +482,35 @@ void reduce_partials(const double *part, uint32_t nblk, int nrhs, double *out, h
 }
 
 // ------------------------------------------------------------ instantiations
-#define KQ_INST_S(S)                                                                         \
-    template void cgm_begin<S>(CgmState<S> *, const CgmLaunch &, hipStream_t);               \
-    template void cgm_scalars1<S>(CgmState<S> *, const CgmLaunch &, hipStream_t);            \
-    template void cgm_scalars2<S>(CgmState<S> *, const CgmLaunch &, hipStream_t);            \
-    template void cgm_init_state<S>(CgmState<S> *, const CgmLaunch &, hipStream_t);
+#define KQ_INST_S(S)                                                                                          \
+    template void cgm_begin<S>(CgmState<S> *, const CgmLaunch &, hipStream_t);                                \
+    template void cgm_scalars1<S>(CgmState<S> *, const CgmLaunch &, hipStream_t);                             \
+    template void cgm_scalars2<S>(CgmState<S> *, const CgmLaunch &, hipStream_t);                             \
+    template void cgm_init_list<S>(CgmState<S> *, const int *, const int *, int, const double *, uint32_t,    \
+                                   hipStream_t);                                                              \
+    template void rel_set_rr<S>(CgmState<S> *, const int *, const int *, int, const double *, uint32_t,       \
+                                hipStream_t);
 KQ_INST_S(double)
 KQ_INST_S(float)
-template void cgm_axpy_norm<double, double>(CgmState<double> *, Vec<double>, Vec<double>, const CgmLaunch &,
-                                            hipStream_t);
-template void cgm_axpy_norm<float, float>(CgmState<float> *, Vec<float>, Vec<float>, const CgmLaunch &,
-                                          hipStream_t);
-template void cgm_axpy_norm<double, float>(CgmState<double> *, Vec<float>, Vec<float>, const CgmLaunch &,
-                                           hipStream_t);
-template void cgm_xp_update<double, double, double>(CgmState<double> *, double *, double *, Vec<double>, int,
-                                                    const CgmLaunch &, hipStream_t);
-template void cgm_xp_update<float, float, float>(CgmState<float> *, float *, float *, Vec<float>, int,
-                                                 const CgmLaunch &, hipStream_t);
-template void cgm_xp_update<double, float, float>(CgmState<double> *, float *, float *, Vec<float>, int,
-                                                  const CgmLaunch &, hipStream_t);
+
+#define KQ_INST_SV(S, VR)                                                                                      \
+    template void cgm_axpy_norm<S, VR>(CgmState<S> *, Vec<VR>, Vec<VR>, const CgmLaunch &, hipStream_t);       \
+    template void cgm_xp_update<S, VR, VR>(CgmState<S> *, VR *, VR *, Vec<VR>, const CgmLaunch &, hipStream_t); \
+    template void rel_fold<S, VR>(CgmState<S> *, double *, VR *, const size_t *, uint32_t, const int *,        \
+                                  const int *, int, hipStream_t);                                             \
+    template void copy_r_to_p<S, VR>(const CgmState<S> *, Vec<VR>, VR *, const size_t *, const int *,          \
+                                     const int *, int, hipStream_t);
+KQ_INST_SV(double, double)
+KQ_INST_SV(float, float)
+KQ_INST_SV(double, float)
+
 template void rel_residual<float>(const double *, const double *, Vec<float>, const int *, const int *, int,
-                                   double *, uint32_t, hipStream_t);
+                                  double *, uint32_t, hipStream_t);
 template void rel_residual<double>(const double *, const double *, Vec<double>, const int *, const int *, int,
                                    double *, uint32_t, hipStream_t);
-template void rel_set_rr<double>(CgmState<double> *, const int *, const int *, int, const double *, uint32_t,
-                                 hipStream_t);
-template void rel_set_rr<float>(CgmState<float> *, const int *, const int *, int, const double *, uint32_t,
-                                hipStream_t);
-template void rel_fold<double, float>(CgmState<double> *, double *, float *, int, uint32_t, const int *, const int *,
-                                      int, hipStream_t);
-template void rel_fold<double, double>(CgmState<double> *, double *, double *, int, uint32_t, const int *,
-                                       const int *, int, hipStream_t);
-template void rel_fold<float, float>(CgmState<float> *, double *, float *, int, uint32_t, const int *, const int *,
-                                     int, hipStream_t);
-template void vec_dot_partial<double>(Vec<double>, Vec<double>, int, double *, uint32_t, hipStream_t);
-template void vec_dot_partial<float>(Vec<float>, Vec<float>, int, double *, uint32_t, hipStream_t);
+template void vec_dot_partial<double>(Vec<double>, Vec<double>, int, double *, uint32_t, hipStream_t, const int *,
+                                      const int *);
+template void vec_dot_partial<float>(Vec<float>, Vec<float>, int, double *, uint32_t, hipStream_t, const int *,
+                                     const int *);
 
 }  // namespace kq

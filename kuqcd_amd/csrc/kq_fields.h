@@ -5,7 +5,8 @@
 // 64 lanes read 64 consecutive sites with one coalesced load per component):
 //   links   : comp[((q*4 + mu)*NC + k)*Vh + s]   q = parity block, NC components
 //   scales  : scale[(q*4 + mu)*Vh + s]           (FP16 formats only)
-//   vectors : v[r*rstride + c*Vh + s]            r = right-hand side, c = colour
+//   vectors : v[off(r) + c*Vh + s]               r = right-hand side, c = colour,
+//             off(r) = r*rstride or a per-right-hand-side offset table
 //
 // Device links are pre-scaled at load time: fat links by 1/2 and long links by
 // c3000 = -1/48 (dirac_hisq.cpp:40, 473, 480), so the hopping term is
@@ -129,17 +130,19 @@ template <typename R> struct Vec {
     using V2 = c2_t<R>;
     V2 *p;
     uint32_t Vh;
-    size_t rstride;  // elements between consecutive right-hand sides (3*Vh when packed)
+    size_t rstride;              // elements between consecutive right-hand sides (3*Vh when packed)
+    const size_t *roff = nullptr;  // per right-hand-side element offsets (device); overrides rstride
+    __device__ inline size_t base(int r) const { return roff ? roff[r] : (size_t)r * rstride; }
     __device__ inline void load(int r, uint32_t s, V2 v[3]) const
     {
-        const V2 *b = p + (size_t)r * rstride + s;
+        const V2 *b = p + base(r) + s;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             v[c] = b[(size_t)c * Vh];
     }
     __device__ inline void store(int r, uint32_t s, const V2 v[3]) const
     {
-        V2 *b = p + (size_t)r * rstride + s;
+        V2 *b = p + base(r) + s;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             b[(size_t)c * Vh] = v[c];

@@ -67,22 +67,22 @@ static KuQCDStatus solve_ref_t(int slot, LK fat, LK lng, void **x, void *b, KuQC
     const int parity = (p.matpc_type == KUQCD_EVEN_EVEN) ? 0 : 1;
     const int ns = p.num_shift;
     RunnerTuning tune;
-    const auto nrnt = tuned_launch(slot, KQ_MODE_REFERENCE, 1);
     tune.nr = 1;
-    tune.nt = nrnt.second;
+    tune.nt = tuned_launch(slot, KQ_MODE_REFERENCE, 1).second;
+    tune.chunk = 16;
     CgmRunner<T, T, LK, LK> run(g, parity, fat, lng, 1, ns, c.stream, tune);
-    DevBuf stage;
+    DevBuf stage, bdev(sizeof(c2_t<T>) * 3 * (size_t)g.Vh);
     const size_t n3 = (size_t)3 * g.Vh;
-    Vec<T> rv{reinterpret_cast<c2_t<T> *>(run.r_buf()), g.Vh, n3};
-    upload_vec<T, T>(b, rv, 0, stage, c.stream);
+    upload_vec<T, T>(b, Vec<T>{bdev.as<c2_t<T>>(), g.Vh, n3}, 0, stage, c.stream);
     std::vector<T> sh(ns);
     for (int k = 0; k < ns; ++k)
         sh[k] = (T)p.shifts[k];  // kuqcd.cpp:270-272
-    run.start(sh, std::vector<double>{p.tol}, p.max_iter);
+    const std::vector<double> tolk(ns, p.tol);
+    run.add_system(ns, sh.data(), tolk.data(), p.max_iter, bdev.as<void>(), nullptr);
     run.run();
-    const int iter = run.states()[0].iter;
+    const int iter = run.state(0).iter;
     for (int k = 0; k < ns; ++k) {
-        Vec<T> xv{reinterpret_cast<c2_t<T> *>(run.x_buf()) + (size_t)k * n3, g.Vh, n3};
+        Vec<T> xv{reinterpret_cast<c2_t<T> *>(run.x_lo(0, k)), g.Vh, n3};
         download_vec<T, T>(xv, 0, x[k], stage, c.stream);
     }
     p.iter = iter;
@@ -177,55 +177,111 @@ struct Problem {
     const KqSolveOptions *o;
     DevBuf *b;  // FP64 [nrhs][3][Vh]
     std::vector<double> b2, tol, eps;
+    double delta = 0.1;
 };
 
+// Batched single-shift refinement of the listed (rhs, shift) pairs from their
+// current FP64 solutions in x_out, to 0.5 * eps relative residual.
+template <class LF, class LL>
+static void refine_pairs(Problem &P, LF fat, LL lng, DevBuf &x_out, const std::vector<std::pair<int, int>> &need,
+                         RunnerTuning tune, KqSolveReport *rep)
+{
+    Context &c = ctx();
+    const size_t n3 = (size_t)3 * P.g.Vh;
+    const int nsys = (int)need.size();
+    tune.nr = P.o->nr > 0 ? P.o->nr : tuned_launch(P.slot, P.o->mode, nsys).first;
+    CgmRunner<double, float, LF, LL> ref(P.g, P.parity, fat, lng, nsys, nsys, c.stream, tune);
+    ref.enable_reliable(link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.delta);
+    for (int i = 0; i < nsys; ++i) {
+        const int r = need[i].first, k = need[i].second;
+        const double sh = abs_shifts(*P.p, r)[k];
+        const double tol = 0.25 * P.eps[r] * P.eps[r] * P.b2[r];
+        ref.add_system(1, &sh, &tol, P.p->max_iter, P.b->as<double>() + (size_t)r * 2 * n3,
+                       x_out.as<double>() + ((size_t)r * P.ns + k) * 2 * n3);
+    }
+    ref.run();
+    for (int i = 0; i < nsys; ++i) {
+        const int r = need[i].first, k = need[i].second;
+        KQ_HIP_CHECK(hipMemcpyAsync(x_out.as<double>() + ((size_t)r * P.ns + k) * 2 * n3, ref.x_hi(i, 0),
+                                    sizeof(double2) * n3, hipMemcpyDeviceToDevice, c.stream));
+        rep[r].refine_iter[k] += ref.state(i).iter;
+    }
+    KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+// Mixed precision: one multishift system per source in the sloppy format with
+// FP64 reliable updates of the base shift. Each shift k >= 1 is forked out of
+// the multishift as soon as its iterated relative residual reaches fork_tol
+// and continues as an independent single-shift reliable CG from its FP64
+// solution, inside the same batched iteration (the links are read once for all
+// live systems). Every shift is then verified in FP64; any that misses the
+// target is refined again.
 template <class LF, class LL>
 static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *rep, std::vector<double> &res2)
 {
     Context &c = ctx();
     const size_t n3 = (size_t)3 * P.g.Vh;
-    const auto nrnt = tuned_launch(P.slot, P.o->mode, P.nrhs);
+    const int ns = P.ns, nrhs = P.nrhs;
     RunnerTuning tune;
-    tune.nr = P.o->nr > 0 ? P.o->nr : nrnt.first;
-    tune.nt = P.o->nt > 0 ? P.o->nt : nrnt.second;
-    tune.chunk = P.o->chunk > 0 ? P.o->chunk : 16;
-    const double delta = P.o->reliable_delta > 0 ? P.o->reliable_delta : 0.1;
-    const LinkD f64 = link_d(c.fat[P.slot]), l64 = link_d(c.lng[P.slot]);
+    tune.nt = P.o->nt > 0 ? P.o->nt : tuned_launch(P.slot, P.o->mode, nrhs).second;
+    tune.chunk = P.o->chunk > 0 ? P.o->chunk : 8;
+    // relative fork threshold; about the sloppy format's accuracy floor by default
+    const double fork = P.o->fork_tol < 0 ? 0.0
+                        : P.o->fork_tol > 0 ? P.o->fork_tol
+                                            : (P.o->mode == KQ_MODE_MIXED_FP16 ? 1e-3 : 1e-5);
+    const int max_sys = nrhs * ns;
+    // nr for the peak number of live systems (all multishift systems plus forks)
+    tune.nr = P.o->nr > 0 ? P.o->nr : tuned_launch(P.slot, P.o->mode, std::min(max_sys, 16)).first;
+    x_out.alloc(sizeof(double2) * nrhs * ns * n3);
 
     auto t0 = clk::now();
-    {
-        CgmRunner<double, float, LF, LL> run(P.g, P.parity, fat, lng, P.nrhs, P.ns, c.stream, tune);
-        run.enable_reliable(f64, l64, delta);
-        KQ_HIP_CHECK(hipMemcpyAsync(run.b_hi(), P.b->as<void>(), sizeof(double2) * P.nrhs * n3,
-                                    hipMemcpyDeviceToDevice, c.stream));
-        std::vector<double> sh((size_t)P.nrhs * P.ns);
-        for (size_t i = 0; i < sh.size(); ++i)
-            sh[i] = P.p->shifts[i];
-        run.start(sh, P.tol, P.p->max_iter);
-        run.run();
-        const auto st = run.states();
-        for (int r = 0; r < P.nrhs; ++r) {
-            if (rep) {
-                rep[r].iter = st[r].iter;
-                rep[r].reliable_updates = run.reliable_updates();
-            }
-        }
-        x_out.alloc(sizeof(double2) * P.nrhs * P.ns * n3);
-        KQ_HIP_CHECK(hipMemcpyAsync(x_out.as<void>(), run.x_hi(), x_out.bytes(), hipMemcpyDeviceToDevice, c.stream));
-        KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+    CgmRunner<double, float, LF, LL> run(P.g, P.parity, fat, lng, max_sys, (size_t)nrhs * ns + (size_t)nrhs * (ns - 1),
+                                         c.stream, tune);
+    run.enable_reliable(link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.delta);
+    std::vector<int> sys_of(nrhs);
+    std::vector<int> rhs_of(max_sys, -1);
+    std::vector<int> forked((size_t)nrhs * ns, -1);
+    for (int r = 0; r < nrhs; ++r) {
+        std::vector<double> tolk(ns, P.tol[r]);
+        if (fork > 0)
+            for (int k = 1; k < ns; ++k)
+                tolk[k] = std::max(P.tol[r], fork * fork * P.b2[r]);
+        sys_of[r] = run.add_system(ns, P.p->shifts + (size_t)r * ns, tolk.data(), P.p->max_iter,
+                                   P.b->as<double>() + (size_t)r * 2 * n3, nullptr);
+        rhs_of[sys_of[r]] = r;
     }
+    auto on_drop = [&](int sys, int k) {
+        const int r = rhs_of[sys];
+        if (r < 0 || fork <= 0)
+            return;
+        const double sh = abs_shifts(*P.p, r)[k];
+        const double tol = 0.25 * P.eps[r] * P.eps[r] * P.b2[r];
+        forked[(size_t)r * ns + k] = run.add_system(1, &sh, &tol, P.p->max_iter, run.b_hi(sys), run.x_hi(sys, k));
+    };
+    run.run(on_drop);
+    for (int r = 0; r < nrhs; ++r) {
+        rep[r].iter = run.state(sys_of[r]).iter;
+        rep[r].reliable_updates = run.reliable_updates();
+        for (int k = 0; k < ns; ++k) {
+            const int f = forked[(size_t)r * ns + k];
+            const double *src = f >= 0 ? run.x_hi(f, 0) : run.x_hi(sys_of[r], k);
+            if (f >= 0)
+                rep[r].refine_iter[k] = run.state(f).iter;
+            KQ_HIP_CHECK(hipMemcpyAsync(x_out.as<double>() + ((size_t)r * ns + k) * 2 * n3, src,
+                                        sizeof(double2) * n3, hipMemcpyDeviceToDevice, c.stream));
+        }
+    }
+    KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
     auto t1 = clk::now();
-    if (rep)
-        for (int r = 0; r < P.nrhs; ++r)
-            rep[r].seconds_multishift = secs(t0, t1);
+    for (int r = 0; r < nrhs; ++r)
+        rep[r].seconds_multishift = secs(t0, t1);
 
-    // per-shift FP64 verification and batched refinement
-    res2.assign((size_t)P.nrhs * P.ns, 0.0);
+    // per-shift FP64 verification; refine what misses the target
+    res2.assign((size_t)nrhs * ns, 0.0);
     double t_verify = 0.0;
-    // verify the listed (rhs, shift) pairs; all of them when the list is empty
     auto verify = [&](const std::vector<std::pair<int, int>> &pairs) {
         auto tv0 = clk::now();
-        for (int r = 0; r < P.nrhs; ++r) {
+        for (int r = 0; r < nrhs; ++r) {
             std::vector<int> ks;
             for (const auto &pr : pairs)
                 if (pr.first == r)
@@ -233,68 +289,40 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
             if (!pairs.empty() && ks.empty())
                 continue;
             const auto v = true_res2(P.slot, P.parity, P.b->as<double>() + (size_t)r * 2 * n3,
-                                     x_out.as<double>() + (size_t)r * P.ns * 2 * n3, P.ns, abs_shifts(*P.p, r), ks);
-            for (int k = 0; k < P.ns; ++k)
+                                     x_out.as<double>() + (size_t)r * ns * 2 * n3, ns, abs_shifts(*P.p, r), ks);
+            for (int k = 0; k < ns; ++k)
                 if (pairs.empty() || std::find(ks.begin(), ks.end(), k) != ks.end())
-                    res2[(size_t)r * P.ns + k] = v[k];
+                    res2[(size_t)r * ns + k] = v[k];
         }
         t_verify += secs(tv0, clk::now());
     };
     verify({});
-    for (int r = 0; r < P.nrhs; ++r)
-        for (int k = 0; k < P.ns; ++k)
-            rep[r].initial_rel_res[k] = std::sqrt(res2[(size_t)r * P.ns + k] / P.b2[r]);
+    for (int r = 0; r < nrhs; ++r)
+        for (int k = 0; k < ns; ++k)
+            rep[r].initial_rel_res[k] = std::sqrt(res2[(size_t)r * ns + k] / P.b2[r]);
     if (debug_on())
-        for (int k = 0; k < P.ns; ++k)
-            fprintf(stderr, "[kq] shift %d initial true rel res %.3e\n", k, rep[0].initial_rel_res[k]);
-    if (!P.o->refine) {
-        for (int r = 0; r < P.nrhs; ++r)
-            rep[r].seconds_verify = t_verify;
-        return;
-    }
-    for (int round = 0; round < 3; ++round) {
-        std::vector<std::pair<int, int>> need;
-        for (int r = 0; r < P.nrhs; ++r)
-            for (int k = 0; k < P.ns; ++k)
-                if (std::sqrt(res2[(size_t)r * P.ns + k] / P.b2[r]) > P.eps[r])
-                    need.emplace_back(r, k);
-        if (need.empty())
-            break;
-        const int nsys = (int)need.size();
-        RunnerTuning t2 = tune;
-        t2.nr = P.o->nr > 0 ? P.o->nr : tuned_launch(P.slot, P.o->mode, nsys).first;
-        CgmRunner<double, float, LF, LL> ref(P.g, P.parity, fat, lng, nsys, 1, c.stream, t2);
-        ref.enable_reliable(f64, l64, delta);
-        std::vector<double> sh(nsys), tol(nsys);
-        for (int i = 0; i < nsys; ++i) {
-            const int r = need[i].first, k = need[i].second;
-            KQ_HIP_CHECK(hipMemcpyAsync(ref.b_hi() + (size_t)i * 2 * n3, P.b->as<double>() + (size_t)r * 2 * n3,
-                                        sizeof(double2) * n3, hipMemcpyDeviceToDevice, c.stream));
-            KQ_HIP_CHECK(hipMemcpyAsync(ref.x_hi() + (size_t)i * 2 * n3,
-                                        x_out.as<double>() + ((size_t)r * P.ns + k) * 2 * n3, sizeof(double2) * n3,
-                                        hipMemcpyDeviceToDevice, c.stream));
-            sh[i] = abs_shifts(*P.p, r)[k];
-            tol[i] = 0.25 * P.eps[r] * P.eps[r] * P.b2[r];
-        }
-        ref.start(sh, tol, P.p->max_iter, true);
-        ref.run();
-        const auto st = ref.states();
-        for (int i = 0; i < nsys; ++i) {
-            const int r = need[i].first, k = need[i].second;
-            KQ_HIP_CHECK(hipMemcpyAsync(x_out.as<double>() + ((size_t)r * P.ns + k) * 2 * n3,
-                                        ref.x_hi() + (size_t)i * 2 * n3, sizeof(double2) * n3,
-                                        hipMemcpyDeviceToDevice, c.stream));
-            if (rep) {
-                rep[r].refine_iter[k] += st[i].iter;
-                rep[r].refine_rounds = round + 1;
-            }
-        }
-        KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
-        verify(need);
-    }
+        for (int k = 0; k < ns; ++k)
+            fprintf(stderr, "[kq] shift %d true rel res after the main phase %.3e (fork iters %d)\n", k,
+                    rep[0].initial_rel_res[k], rep[0].refine_iter[k]);
     auto t2 = clk::now();
-    for (int r = 0; r < P.nrhs; ++r) {
-        rep[r].seconds_refine = secs(t1, t2) - t_verify;
+    if (P.o->refine) {
+        for (int round = 0; round < 3; ++round) {
+            std::vector<std::pair<int, int>> need;
+            for (int r = 0; r < nrhs; ++r)
+                for (int k = 0; k < ns; ++k)
+                    if (!(std::sqrt(res2[(size_t)r * ns + k] / P.b2[r]) <= P.eps[r]))
+                        need.emplace_back(r, k);
+            if (need.empty())
+                break;
+            refine_pairs<LF, LL>(P, fat, lng, x_out, need, tune, rep);
+            for (int r = 0; r < nrhs; ++r)
+                rep[r].refine_rounds = round + 1;
+            verify(need);
+        }
+    }
+    auto t3 = clk::now();
+    for (int r = 0; r < nrhs; ++r) {
+        rep[r].seconds_refine = secs(t2, t3) - (t_verify - 0.0);
         rep[r].seconds_verify = t_verify;
     }
 }
@@ -310,24 +338,24 @@ static void run_fp64(Problem &P, DevBuf &x_out, KqSolveReport *rep, std::vector<
     tune.chunk = P.o->chunk > 0 ? P.o->chunk : 16;
     auto t0 = clk::now();
     CgmRunner<double, double, LinkD, LinkD> run(P.g, P.parity, link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.nrhs,
-                                                P.ns, c.stream, tune);
-    KQ_HIP_CHECK(hipMemcpyAsync(run.r_buf(), P.b->as<void>(), sizeof(double2) * P.nrhs * n3, hipMemcpyDeviceToDevice,
-                                c.stream));
-    std::vector<double> sh((size_t)P.nrhs * P.ns);
-    for (size_t i = 0; i < sh.size(); ++i)
-        sh[i] = P.p->shifts[i];
-    run.start(sh, P.tol, P.p->max_iter);
+                                                (size_t)P.nrhs * P.ns, c.stream, tune);
+    for (int r = 0; r < P.nrhs; ++r) {
+        std::vector<double> tolk(P.ns, P.tol[r]);
+        run.add_system(P.ns, P.p->shifts + (size_t)r * P.ns, tolk.data(), P.p->max_iter,
+                       P.b->as<double>() + (size_t)r * 2 * n3, nullptr);
+    }
     run.run();
-    const auto st = run.states();
     x_out.alloc(sizeof(double2) * P.nrhs * P.ns * n3);
-    KQ_HIP_CHECK(hipMemcpyAsync(x_out.as<void>(), run.x_buf(), x_out.bytes(), hipMemcpyDeviceToDevice, c.stream));
+    for (int r = 0; r < P.nrhs; ++r)
+        for (int k = 0; k < P.ns; ++k)
+            KQ_HIP_CHECK(hipMemcpyAsync(x_out.as<double>() + ((size_t)r * P.ns + k) * 2 * n3, run.x_lo(r, k),
+                                        sizeof(double2) * n3, hipMemcpyDeviceToDevice, c.stream));
     KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
     auto t1 = clk::now();
-    if (rep)
-        for (int r = 0; r < P.nrhs; ++r) {
-            rep[r].iter = st[r].iter;
-            rep[r].seconds_multishift = secs(t0, t1);
-        }
+    for (int r = 0; r < P.nrhs; ++r) {
+        rep[r].iter = run.state(r).iter;
+        rep[r].seconds_multishift = secs(t0, t1);
+    }
     res2.assign((size_t)P.nrhs * P.ns, 0.0);
     auto tv0 = clk::now();
     if (P.o->refine)
@@ -382,6 +410,7 @@ KuQCDStatus solve_extended(int slot, int nrhs, void **x, void **b, KuQCDInvertPa
     P.g = slot_geom(slot);
     P.p = &p;
     P.o = &o;
+    P.delta = o.reliable_delta > 0 ? o.reliable_delta : 0.1;
     const size_t n3 = (size_t)3 * P.g.Vh;
     DevBuf bdev(sizeof(double2) * nrhs * n3), stage;
     for (int r = 0; r < nrhs; ++r) {

@@ -1,13 +1,17 @@
 // Batched multishift CG on the device: the host side of CGM::Invert
-// (inverter_cgm.cpp:17-87) for nsys independent systems that share one gauge
-// field, each with its own list of shifts.
+// (inverter_cgm.cpp:17-87) for many independent systems that share one gauge
+// field, each with its own list of shifts (a multishift system) or a single
+// shift (a refinement system started from an initial guess).
 //
 // The iteration is captured once in a hipGraph and replayed in chunks; all
 // recurrence scalars live in device memory (CgmState), so a chunk needs no host
-// round trip. Between chunks the host reads the states back, and in mixed
-// precision performs reliable updates: the FP32 iterated solution is folded
-// into FP64 accumulators and the residual of the base system is recomputed in
-// FP64, r = b - A_0 x, whenever |r| has dropped by `delta` since the last one.
+// round trip, and one batched hopping-term launch per half-iteration serves
+// every live system, so each link is read from HBM once for all of them.
+// Between chunks the host reads the states back, and in mixed precision
+// performs reliable updates: the FP32 iterated solutions are folded into FP64
+// accumulators and the base residual is recomputed in FP64, r = b - A_0 x,
+// whenever |r| has dropped by `delta` since the last one. New systems may be
+// added between chunks (e.g. when a multishift system drops a shift).
 #pragma once
 
 #include <algorithm>
@@ -15,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "kq_cgm_kernels.h"
@@ -107,188 +112,201 @@ private:
     size_t n_ = 0;
 };
 
+
 struct RunnerTuning {
-    int nr = 1;       // right-hand sides per thread in the batched hopping term
-    int nt = 256;     // threads per block of the hopping term
-    int chunk = 16;   // iterations replayed between host checks
+    int nr = 1;      // right-hand sides per thread in the batched hopping term
+    int nt = 256;    // threads per block of the hopping term
+    int chunk = 8;   // iterations replayed between host checks
 };
 
+// Batched multishift CG over up to max_sys systems that share one gauge field.
+// S: recurrence scalar type; VR: iteration-vector real type; LF/LL: link
+// formats of the iteration. In reliable mode (VR sloppy) every system also
+// carries an FP64 source and FP64 solutions, and its base residual is
+// recomputed in FP64 whenever |r| dropped by delta since the last time.
 template <typename S, typename VR, class LF, class LL> class CgmRunner {
 public:
     using V2 = c2_t<VR>;
+    using DropFn = std::function<void(int sys, int k)>;
 
-    CgmRunner(const Geom &g, int parity, LF fat, LL lng, int nsys, int nshift, hipStream_t st, RunnerTuning tune)
-        : g_(g), parity_(parity), fat_(fat), lng_(lng), nsys_(nsys), nshift_(nshift), st_(st), tune_(tune)
+    CgmRunner(const Geom &g, int parity, LF fat, LL lng, int max_sys, size_t pool_vecs, hipStream_t st,
+              RunnerTuning tune)
+        : g_(g), parity_(parity), fat_(fat), lng_(lng), max_sys_(max_sys), pool_vecs_(pool_vecs), st_(st), tune_(tune)
     {
-        const size_t n3 = (size_t)3 * g.Vh;
-        r_.alloc(sizeof(V2) * nsys * n3);
-        s_.alloc(sizeof(V2) * nsys * n3);
-        tmp_.alloc(sizeof(V2) * nsys * n3);
-        p_.alloc(sizeof(V2) * nsys * nshift * n3);
-        x_.alloc(sizeof(V2) * nsys * nshift * n3);
-        state_.alloc(sizeof(CgmState<S>) * nsys);
-        state_h_.alloc(nsys);
+        n3_ = (size_t)3 * g.Vh;
+        r_.alloc(sizeof(V2) * max_sys * n3_);
+        s_.alloc(sizeof(V2) * max_sys * n3_);
+        tmp_.alloc(sizeof(V2) * max_sys * n3_);
+        p_.alloc(sizeof(V2) * pool_vecs * n3_);
+        x_.alloc(sizeof(V2) * pool_vecs * n3_);
+        state_.alloc(sizeof(CgmState<S>) * max_sys);
+        state_h_.alloc(max_sys);
+        std::memset(state_h_.get(), 0, sizeof(CgmState<S>) * max_sys);
+        for (int i = 0; i < max_sys; ++i)
+            state_h_[i].stop = 1;  // unused slots never run
         nblk_ = blas_nblk(g.Vh);
         nblk_dsl_ = dslash_nblk(g.Vh, tune.nt);
-        part_a_.alloc(sizeof(double) * nsys * nblk_dsl_);
-        part_b_.alloc(sizeof(double) * nsys * nblk_);
-        ints_.alloc(sizeof(int) * (2 * nsys + 4));
-        sigma_.alloc(sizeof(double) * nsys);
-        last_fold_.assign(nsys, nshift);
+        part_a_.alloc(sizeof(double) * max_sys * std::max(nblk_dsl_, dslash_nblk(g.Vh, 256)));
+        part_b_.alloc(sizeof(double) * max_sys * nblk_);
+        ints_.alloc(sizeof(int) * (2 * max_sys + 4));
+        sigma_.alloc(sizeof(double) * max_sys);
+        off_.alloc(sizeof(size_t) * max_sys);
+        off_h_.assign(max_sys, 0);
+        sig_h_.assign(max_sys, 0.0);
+        last_fold_.assign(max_sys, 0);
+        prev_max_.assign(max_sys, 0);
+        finalized_.assign(max_sys, 1);
+        KQ_HIP_CHECK(hipMemcpyAsync(state_.as<void>(), state_h_.get(), sizeof(CgmState<S>) * max_sys,
+                                    hipMemcpyHostToDevice, st_));
+        KQ_HIP_CHECK(hipStreamSynchronize(st_));
     }
     ~CgmRunner()
     {
         if (graph_exec_)
             (void)hipGraphExecDestroy(graph_exec_);
     }
+    CgmRunner(const CgmRunner &) = delete;
+    CgmRunner &operator=(const CgmRunner &) = delete;
 
     // Mixed precision: FP64 links for the true residual and FP64 accumulators.
-    void enable_reliable(LinkD fat64, LinkD lng64, double delta)
+    // A system pauses (waits for its reliable update) once |r|^2 falls below
+    // pause_factor * delta^2 * (|r|^2 at the last update).
+    void enable_reliable(LinkD fat64, LinkD lng64, double delta, double pause_factor = 1e-2)
     {
         reliable_ = true;
         fat64_ = fat64;
         lng64_ = lng64;
         delta_ = delta;
-        const size_t n3 = (size_t)3 * g_.Vh;
-        b_hi_.alloc(sizeof(double2) * nsys_ * n3);
-        x_hi_.alloc(sizeof(double2) * nsys_ * nshift_ * n3);
-        tmp64_.alloc(sizeof(double2) * nsys_ * n3);
-        ax64_.alloc(sizeof(double2) * nsys_ * n3);
-        nblk_dsl64_ = dslash_nblk(g_.Vh, 256);
-        part_rel_.alloc(sizeof(double) * nsys_ * nblk_dsl64_);
+        pause2_ = pause_factor * delta * delta;
+        b_hi_.alloc(sizeof(double2) * max_sys_ * n3_);
+        x_hi_.alloc(sizeof(double2) * pool_vecs_ * n3_);
+        tmp64_.alloc(sizeof(double2) * max_sys_ * n3_);
+        ax64_.alloc(sizeof(double2) * max_sys_ * n3_);
     }
 
-    // Buffers the caller fills before start():
-    VR *r_buf() const { return r_.as<VR>(); }              // [nsys][3][Vh] source (plain mode)
-    double *b_hi() const { return b_hi_.as<double>(); }    // [nsys][3][Vh] FP64 source (reliable mode)
-    double *x_hi() const { return x_hi_.as<double>(); }    // [nsys][nshift][3][Vh] FP64 solution
-    VR *x_buf() const { return x_.as<VR>(); }              // [nsys][nshift][3][Vh] iterated solution
-    const Geom &geom() const { return g_; }
-    int nsys() const { return nsys_; }
-    int nshift() const { return nshift_; }
-
-    // shifts: nsys*nshift in the reference convention (absolute shift[0],
-    // shift[k>=1] relative to it); tol: absolute squared target per system.
-    // initial_guess (reliable mode, nshift == 1 only): start from x_hi.
-    void start(const std::vector<S> &shifts, const std::vector<double> &tol, int max_iter, bool initial_guess = false)
+    // Register a system; it starts at the next run() step. shifts: reference
+    // convention (absolute shifts[0], shifts[k>=1] relative to it); tolk: absolute
+    // squared target per shift. b: device source, FP64 SoA in reliable mode, VR
+    // SoA otherwise. x0 (reliable mode, single shift): FP64 initial guess or null.
+    int add_system(int nshift, const S *shifts, const double *tolk, int max_iter, const void *b, const double *x0)
     {
-        if (nshift_ > kMaxShift)
-            throw std::invalid_argument("too many shifts");
-        if (initial_guess && (!reliable_ || nshift_ != 1))
-            throw std::invalid_argument("an initial guess needs the reliable single-shift mode");
-        const size_t n3 = (size_t)3 * g_.Vh;
-        std::vector<double> sig(nsys_);
-        for (int r = 0; r < nsys_; ++r) {
-            CgmState<S> &c = state_h_[r];
-            std::memset(&c, 0, sizeof(c));
-            c.nshift = nshift_;
-            c.maxIter = max_iter;
-            c.tol = tol[r];
-            for (int k = 0; k < nshift_; ++k)
-                c.shift[k] = shifts[(size_t)r * nshift_ + k];
-            sig[r] = (double)shifts[(size_t)r * nshift_];
+        if (nsys_ >= max_sys_)
+            throw std::runtime_error("solver: no free system slot");
+        if (nshift < 1 || nshift > kMaxShift)
+            throw std::invalid_argument("solver: bad shift count");
+        if (pool_used_ + (size_t)nshift > pool_vecs_)
+            throw std::runtime_error("solver: vector pool exhausted");
+        if (x0 && (!reliable_ || nshift != 1))
+            throw std::invalid_argument("solver: an initial guess needs the reliable single-shift mode");
+        const int sys = nsys_++;
+        off_h_[sys] = pool_used_ * n3_;
+        pool_used_ += nshift;
+        CgmState<S> &c = state_h_[sys];
+        std::memset(&c, 0, sizeof(c));
+        c.nshift = nshift;
+        c.maxIter = max_iter;
+        c.stop = 1;  // until started
+        c.pause2 = reliable_ ? pause2_ : 0.0;
+        for (int k = 0; k < nshift; ++k) {
+            c.shift[k] = shifts[k];
+            c.tolk[k] = tolk[k];
         }
-        KQ_HIP_CHECK(hipMemcpyAsync(state_.as<void>(), state_h_.get(), sizeof(CgmState<S>) * nsys_,
-                                    hipMemcpyHostToDevice, st_));
-        KQ_HIP_CHECK(hipMemcpyAsync(sigma_.as<void>(), sig.data(), sizeof(double) * nsys_, hipMemcpyHostToDevice, st_));
-        KQ_HIP_CHECK(hipMemsetAsync(x_.as<void>(), 0, x_.bytes(), st_));
-        set_all_list();
+        sig_h_[sys] = (double)shifts[0];
         if (reliable_) {
-            if (!initial_guess) {
-                KQ_HIP_CHECK(hipMemsetAsync(x_hi_.as<void>(), 0, x_hi_.bytes(), st_));
-                KQ_HIP_CHECK(hipMemsetAsync(ax64_.as<void>(), 0, ax64_.bytes(), st_));
+            KQ_HIP_CHECK(hipMemcpyAsync(b_hi_.as<double2>() + sys * n3_, b, sizeof(double2) * n3_,
+                                        hipMemcpyDeviceToDevice, st_));
+            if (x0) {
+                KQ_HIP_CHECK(hipMemcpyAsync(x_hi_.as<double2>() + off_h_[sys], x0, sizeof(double2) * n3_,
+                                            hipMemcpyDeviceToDevice, st_));
+                pending_x0_.push_back(sys);
             } else {
-                apply_a64_x0();
+                KQ_HIP_CHECK(hipMemsetAsync(x_hi_.as<double2>() + off_h_[sys], 0, sizeof(double2) * nshift * n3_, st_));
+                KQ_HIP_CHECK(hipMemsetAsync(ax64_.as<double2>() + sys * n3_, 0, sizeof(double2) * n3_, st_));
             }
-            rel_residual<VR>(b_hi_.as<double>(), ax64_.as<double>(), vec(r_), rel_list(), rel_n(), nsys_,
-                             part_b_.as<double>(), nblk_, st_);
         } else {
-            vec_dot_partial<VR>(vec(r_), vec(r_), nsys_, part_b_.as<double>(), nblk_, st_);
+            KQ_HIP_CHECK(hipMemcpyAsync(r_.as<V2>() + sys * n3_, b, sizeof(V2) * n3_, hipMemcpyDeviceToDevice, st_));
         }
-        for (int r = 0; r < nsys_; ++r)
-            for (int k = 0; k < nshift_; ++k)
-                KQ_HIP_CHECK(hipMemcpyAsync(p_.as<V2>() + ((size_t)r * nshift_ + k) * n3, r_.as<V2>() + (size_t)r * n3,
-                                            sizeof(V2) * n3, hipMemcpyDeviceToDevice, st_));
-        cgm_init_state<S>(state_.as<CgmState<S>>(), launch(), st_);
-        last_fold_.assign(nsys_, nshift_);
-        reliable_updates_ = 0;
+        KQ_HIP_CHECK(hipMemsetAsync(x_.as<V2>() + off_h_[sys], 0, sizeof(V2) * nshift * n3_, st_));
+        pending_.push_back(sys);
+        return sys;
     }
 
-    // Iterate every system to convergence (or max_iter).
-    void run()
+    // Iterate until every registered system has stopped. on_drop(sys, k) is
+    // called (between chunks) for each shift k >= 1 that a system drops; by
+    // then its solution is final in x_hi(sys, k) (reliable) or x_lo(sys, k).
+    void run(const DropFn &on_drop = nullptr)
     {
+        start_pending();
         if (!graph_exec_)
             capture();
-        int chunk = std::max(1, tune_.chunk);
+        static const bool dbg = getenv("KQ_DEBUG") != nullptr;
         for (;;) {
-            for (int c = 0; c < chunk; ++c)
+            for (int c = 0; c < tune_.chunk; ++c)
                 KQ_HIP_CHECK(hipGraphLaunch(graph_exec_, st_));
-            KQ_HIP_CHECK(hipMemcpyAsync(state_h_.get(), state_.as<void>(), sizeof(CgmState<S>) * nsys_,
-                                        hipMemcpyDeviceToHost, st_));
-            KQ_HIP_CHECK(hipStreamSynchronize(st_));
+            download_states();
+            std::vector<int> fold, rel;
+            std::vector<std::pair<int, int>> drops;
             bool live = false;
-            std::vector<int> rel;
-            static const bool dbg = getenv("KQ_DEBUG") != nullptr;
-            for (int r = 0; r < nsys_; ++r) {
-                const CgmState<S> &c = state_h_[r];
+            for (int sys = 0; sys < nsys_; ++sys) {
+                CgmState<S> &c = state_h_[sys];
+                if (finalized_[sys])
+                    continue;
                 if (dbg)
-                    fprintf(stderr, "[kq] sys %d iter %d rr %.4e r2max %.4e maxTerm %d stop %d B0 %.4e a0 %.4e\n", r,
-                            c.iter, (double)c.rr, c.r2max, c.maxTerm, c.stop, (double)c.B[0], (double)c.a[0]);
-                if (!c.stop)
-                    live = true;
-                if (reliable_) {
-                    const bool drop = (double)c.rr < delta_ * delta_ * c.r2max;
-                    if (c.stop || drop)
-                        rel.push_back(r);
+                    fprintf(stderr, "[kq] sys %d iter %d rr %.4e r2max %.4e maxTerm %d stop %d pause %d\n", sys, c.iter,
+                            (double)c.rr, c.r2max, c.maxTerm, c.stop, c.pause);
+                const bool dropped = c.maxTerm < prev_max_[sys];
+                for (int k = std::max(1, c.maxTerm); k < prev_max_[sys]; ++k)
+                    drops.emplace_back(sys, k);
+                prev_max_[sys] = std::max(0, c.maxTerm);
+                if (c.stop) {
+                    finalized_[sys] = 1;
+                    if (reliable_) {
+                        c.maxTermFold = last_fold_[sys];
+                        fold.push_back(sys);
+                    }
+                    continue;
+                }
+                live = true;
+                const bool need_rel = reliable_ && (c.pause || (double)c.rr < delta_ * delta_ * c.r2max);
+                if (reliable_ && (need_rel || dropped)) {
+                    c.maxTermFold = last_fold_[sys];
+                    fold.push_back(sys);
+                }
+                if (need_rel) {
+                    c.pause = 0;
+                    rel.push_back(sys);
                 }
             }
-            if (reliable_ && !rel.empty())
-                reliable_update(rel);
+            if (!fold.empty() || !rel.empty())
+                reliable_update(fold, rel);
+            if (on_drop)
+                for (const auto &d : drops)
+                    on_drop(d.first, d.second);
+            if (!pending_.empty()) {
+                start_pending();
+                live = true;
+            }
             if (!live)
                 break;
         }
     }
 
-    std::vector<CgmState<S>> states()
-    {
-        KQ_HIP_CHECK(hipMemcpyAsync(state_h_.get(), state_.as<void>(), sizeof(CgmState<S>) * nsys_,
-                                    hipMemcpyDeviceToHost, st_));
-        KQ_HIP_CHECK(hipStreamSynchronize(st_));
-        return std::vector<CgmState<S>>(state_h_.get(), state_h_.get() + nsys_);
-    }
+    double *x_hi(int sys, int k) const { return x_hi_.as<double>() + 2 * (off_h_[sys] + (size_t)k * n3_); }
+    VR *x_lo(int sys, int k) const { return x_.as<VR>() + 2 * (off_h_[sys] + (size_t)k * n3_); }
+    double *b_hi(int sys) const { return b_hi_.as<double>() + 2 * sys * n3_; }
+    const CgmState<S> &state(int sys) const { return state_h_[sys]; }
+    int nsys() const { return nsys_; }
     int reliable_updates() const { return reliable_updates_; }
-
-    // One iteration, launched on the runner's stream (also what the graph holds).
-    void iteration()
-    {
-        const CgmLaunch L = launch();
-        CgmState<S> *st = state_.as<CgmState<S>>();
-        cgm_begin<S>(st, L, st_);
-        DslashArgs<LF, LL, VR> d1 = dargs(1 - parity_);
-        d1.src = vec_p0();
-        d1.dst = vec(tmp_);
-        launch_dslash(d1, EPI_STORE, tune_.nr, tune_.nt, st_);
-        DslashArgs<LF, LL, VR> d2 = dargs(parity_);
-        d2.src = vec(tmp_);
-        d2.dst = vec(s_);
-        d2.aux = vec_p0();
-        d2.sigma = sigma_.as<double>();
-        d2.partial = part_a_.as<double>();
-        launch_dslash(d2, EPI_AXMY_DOT, tune_.nr, tune_.nt, st_);
-        cgm_scalars1<S>(st, L, st_);
-        cgm_axpy_norm<S, VR>(st, vec(r_), vec(s_), L, st_);
-        cgm_scalars2<S>(st, L, st_);
-        cgm_xp_update<S, VR, VR>(st, x_.as<VR>(), p_.as<VR>(), vec(r_), nshift_, L, st_);
-    }
+    const Geom &geom() const { return g_; }
 
 private:
-    Vec<VR> vec(const DevBuf &b) const { return Vec<VR>{b.as<V2>(), g_.Vh, (size_t)3 * g_.Vh}; }
-    Vec<VR> vec_p0() const { return Vec<VR>{p_.as<V2>(), g_.Vh, (size_t)3 * g_.Vh * nshift_}; }
+    Vec<VR> vec(const DevBuf &b) const { return Vec<VR>{b.as<V2>(), g_.Vh, n3_}; }
+    Vec<VR> vec_p0() const { return Vec<VR>{p_.as<V2>(), g_.Vh, 0, off_.as<size_t>()}; }
     int *rlist() const { return ints_.as<int>(); }
-    int *nact() const { return ints_.as<int>() + nsys_; }
-    int *any() const { return ints_.as<int>() + nsys_ + 1; }
-    int *rel_list() const { return ints_.as<int>() + nsys_ + 2; }
-    int *rel_n() const { return ints_.as<int>() + 2 * nsys_ + 2; }
+    int *nact() const { return ints_.as<int>() + max_sys_; }
+    int *any() const { return ints_.as<int>() + max_sys_ + 1; }
+    int *list() const { return ints_.as<int>() + max_sys_ + 2; }
+    int *list_n() const { return ints_.as<int>() + 2 * max_sys_ + 2; }
 
     CgmLaunch launch() const
     {
@@ -296,59 +314,50 @@ private:
         L.Vh = g_.Vh;
         L.nblk = nblk_;
         L.nblk_dsl = nblk_dsl_;
-        L.nrhs = nsys_;
+        L.nsys = max_sys_;
         L.rlist = rlist();
         L.nact = nact();
         L.any = any();
         L.part_a = part_a_.as<double>();
         L.part_b = part_b_.as<double>();
+        L.off = off_.as<size_t>();
         return L;
     }
 
-    DslashArgs<LF, LL, VR> dargs(int dst_parity) const
+    void download_states()
     {
-        DslashArgs<LF, LL, VR> a{};
-        a.g = g_;
-        a.dst_parity = dst_parity;
-        a.fat = fat_;
-        a.lng = lng_;
-        a.run = any();
-        a.rlist = rlist();
-        a.nact = nact();
-        a.nrhs = nsys_;
-        return a;
+        KQ_HIP_CHECK(hipMemcpyAsync(state_h_.get(), state_.as<void>(), sizeof(CgmState<S>) * max_sys_,
+                                    hipMemcpyDeviceToHost, st_));
+        KQ_HIP_CHECK(hipStreamSynchronize(st_));
     }
-
-    // upload a list of systems for the reliable-update kernels (blocking)
+    void upload_states()
+    {
+        KQ_HIP_CHECK(hipMemcpyAsync(state_.as<void>(), state_h_.get(), sizeof(CgmState<S>) * max_sys_,
+                                    hipMemcpyHostToDevice, st_));
+    }
+    // upload a list of systems for the list-driven kernels (blocking)
     void set_list(const std::vector<int> &l)
     {
         const int n = (int)l.size();
         if (n > 0)
-            KQ_HIP_CHECK(hipMemcpyAsync(rel_list(), l.data(), sizeof(int) * n, hipMemcpyHostToDevice, st_));
-        KQ_HIP_CHECK(hipMemcpyAsync(rel_n(), &n, sizeof(int), hipMemcpyHostToDevice, st_));
+            KQ_HIP_CHECK(hipMemcpyAsync(list(), l.data(), sizeof(int) * n, hipMemcpyHostToDevice, st_));
+        KQ_HIP_CHECK(hipMemcpyAsync(list_n(), &n, sizeof(int), hipMemcpyHostToDevice, st_));
         KQ_HIP_CHECK(hipStreamSynchronize(st_));
     }
-    void set_all_list()
-    {
-        std::vector<int> l(nsys_);
-        for (int r = 0; r < nsys_; ++r)
-            l[r] = r;
-        set_list(l);
-    }
 
-    // ax64 = A_0 x_hi[.][0] for the systems of rel_list (FP64 links).
+    // ax64 = A_0 x_hi[.][0] for the listed systems (FP64 links)
     void apply_a64_x0()
     {
         DslashArgs<LinkD, LinkD, double> a{};
         a.g = g_;
         a.fat = fat64_;
         a.lng = lng64_;
-        a.rlist = rel_list();
-        a.nact = rel_n();
-        a.nrhs = nsys_;
-        Vec<double> x0{x_hi_.as<double2>(), g_.Vh, (size_t)3 * g_.Vh * nshift_};
-        Vec<double> t{tmp64_.as<double2>(), g_.Vh, (size_t)3 * g_.Vh};
-        Vec<double> ax{ax64_.as<double2>(), g_.Vh, (size_t)3 * g_.Vh};
+        a.rlist = list();
+        a.nact = list_n();
+        a.nrhs = max_sys_;
+        Vec<double> x0{x_hi_.as<double2>(), g_.Vh, 0, off_.as<size_t>()};
+        Vec<double> t{tmp64_.as<double2>(), g_.Vh, n3_};
+        Vec<double> ax{ax64_.as<double2>(), g_.Vh, n3_};
         a.dst_parity = 1 - parity_;
         a.src = x0;
         a.dst = t;
@@ -358,35 +367,94 @@ private:
         a.dst = ax;
         a.aux = x0;
         a.sigma = sigma_.as<double>();
-        a.partial = part_rel_.as<double>();
+        a.partial = part_a_.as<double>();
         launch_dslash(a, EPI_AXMY_DOT, 1, 256, st_);
     }
 
-    void reliable_update(const std::vector<int> &sys)
+    void start_pending()
     {
-        // fold the FP32 solutions into the FP64 accumulators
-        for (int r : sys)
-            state_h_[r].maxTermFold = last_fold_[r];
-        KQ_HIP_CHECK(hipMemcpyAsync(state_.as<void>(), state_h_.get(), sizeof(CgmState<S>) * nsys_,
-                                    hipMemcpyHostToDevice, st_));
-        set_list(sys);
-        rel_fold<S, VR>(state_.as<CgmState<S>>(), x_hi_.as<double>(), x_.as<VR>(), nshift_, g_.Vh, rel_list(), rel_n(),
-                        nsys_, st_);
-        std::vector<int> live;
-        for (int r : sys) {
-            last_fold_[r] = std::max(0, state_h_[r].maxTerm);
-            if (!state_h_[r].stop)
-                live.push_back(r);
+        if (pending_.empty())
+            return;
+        for (int sys : pending_) {
+            last_fold_[sys] = state_h_[sys].nshift;
+            prev_max_[sys] = state_h_[sys].nshift;
+            finalized_[sys] = 0;
         }
-        if (!live.empty()) {
-            set_list(live);
-            apply_a64_x0();
-            rel_residual<VR>(b_hi_.as<double>(), ax64_.as<double>(), vec(r_), rel_list(), rel_n(), nsys_,
+        upload_states();
+        KQ_HIP_CHECK(hipMemcpyAsync(off_.as<void>(), off_h_.data(), sizeof(size_t) * max_sys_, hipMemcpyHostToDevice,
+                                    st_));
+        KQ_HIP_CHECK(hipMemcpyAsync(sigma_.as<void>(), sig_h_.data(), sizeof(double) * max_sys_,
+                                    hipMemcpyHostToDevice, st_));
+        if (reliable_) {
+            if (!pending_x0_.empty()) {
+                set_list(pending_x0_);
+                apply_a64_x0();
+            }
+            set_list(pending_);
+            rel_residual<VR>(b_hi_.as<double>(), ax64_.as<double>(), vec(r_), list(), list_n(), max_sys_,
                              part_b_.as<double>(), nblk_, st_);
-            rel_set_rr<S>(state_.as<CgmState<S>>(), rel_list(), rel_n(), nsys_, part_b_.as<double>(), nblk_, st_);
+        } else {
+            set_list(pending_);
+            vec_dot_partial<VR>(vec(r_), vec(r_), max_sys_, part_b_.as<double>(), nblk_, st_, list(), list_n());
+        }
+        copy_r_to_p<S, VR>(state_.as<CgmState<S>>(), vec(r_), p_.as<VR>(), off_.as<size_t>(), list(), list_n(),
+                           max_sys_, st_);
+        cgm_init_list<S>(state_.as<CgmState<S>>(), list(), list_n(), max_sys_, part_b_.as<double>(), nblk_, st_);
+        pending_.clear();
+        pending_x0_.clear();
+        download_states();
+    }
+
+    void reliable_update(const std::vector<int> &fold, const std::vector<int> &rel)
+    {
+        upload_states();  // maxTermFold and cleared pause flags
+        if (!fold.empty()) {
+            set_list(fold);
+            rel_fold<S, VR>(state_.as<CgmState<S>>(), x_hi_.as<double>(), x_.as<VR>(), off_.as<size_t>(), g_.Vh,
+                            list(), list_n(), max_sys_, st_);
+            for (int sys : fold)
+                last_fold_[sys] = state_h_[sys].stop ? 0 : std::max(0, state_h_[sys].maxTerm);
+        }
+        if (!rel.empty()) {
+            set_list(rel);
+            apply_a64_x0();
+            rel_residual<VR>(b_hi_.as<double>(), ax64_.as<double>(), vec(r_), list(), list_n(), max_sys_,
+                             part_b_.as<double>(), nblk_, st_);
+            rel_set_rr<S>(state_.as<CgmState<S>>(), list(), list_n(), max_sys_, part_b_.as<double>(), nblk_, st_);
             reliable_updates_ += 1;
         }
-        KQ_HIP_CHECK(hipStreamSynchronize(st_));
+        download_states();  // keep the host copy current (rr, r2max set on the device)
+    }
+
+    // One iteration on the runner's stream (what the graph holds).
+    void iteration()
+    {
+        const CgmLaunch L = launch();
+        CgmState<S> *st = state_.as<CgmState<S>>();
+        cgm_begin<S>(st, L, st_);
+        DslashArgs<LF, LL, VR> d{};
+        d.g = g_;
+        d.fat = fat_;
+        d.lng = lng_;
+        d.run = any();
+        d.rlist = rlist();
+        d.nact = nact();
+        d.nrhs = max_sys_;
+        d.dst_parity = 1 - parity_;
+        d.src = vec_p0();
+        d.dst = vec(tmp_);
+        launch_dslash(d, EPI_STORE, tune_.nr, tune_.nt, st_);
+        d.dst_parity = parity_;
+        d.src = vec(tmp_);
+        d.dst = vec(s_);
+        d.aux = vec_p0();
+        d.sigma = sigma_.as<double>();
+        d.partial = part_a_.as<double>();
+        launch_dslash(d, EPI_AXMY_DOT, tune_.nr, tune_.nt, st_);
+        cgm_scalars1<S>(st, L, st_);
+        cgm_axpy_norm<S, VR>(st, vec(r_), vec(s_), L, st_);
+        cgm_scalars2<S>(st, L, st_);
+        cgm_xp_update<S, VR, VR>(st, x_.as<VR>(), p_.as<VR>(), vec(r_), L, st_);
     }
 
     void capture()
@@ -403,17 +471,25 @@ private:
     int parity_;
     LF fat_;
     LL lng_;
-    int nsys_, nshift_;
+    int max_sys_;
+    size_t pool_vecs_;
     hipStream_t st_;
     RunnerTuning tune_;
-    uint32_t nblk_ = 0, nblk_dsl_ = 0, nblk_dsl64_ = 0;
-    DevBuf r_, s_, tmp_, p_, x_, state_, part_a_, part_b_, ints_, sigma_;
+    size_t n3_ = 0;
+    uint32_t nblk_ = 0, nblk_dsl_ = 0;
+    DevBuf r_, s_, tmp_, p_, x_, state_, part_a_, part_b_, ints_, sigma_, off_;
     Pinned<CgmState<S>> state_h_;
+    std::vector<size_t> off_h_;
+    std::vector<double> sig_h_;
+    int nsys_ = 0;
+    size_t pool_used_ = 0;
+    std::vector<int> pending_, pending_x0_;
+    std::vector<int> last_fold_, prev_max_;
+    std::vector<char> finalized_;
     bool reliable_ = false;
     LinkD fat64_{}, lng64_{};
-    double delta_ = 0.1;
-    DevBuf b_hi_, x_hi_, tmp64_, ax64_, part_rel_;
-    std::vector<int> last_fold_;
+    double delta_ = 0.1, pause2_ = 0.0;
+    DevBuf b_hi_, x_hi_, tmp64_, ax64_;
     int reliable_updates_ = 0;
     hipGraphExec_t graph_exec_ = nullptr;
 };
