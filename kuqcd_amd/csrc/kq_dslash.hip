@@ -12,9 +12,18 @@
 
 namespace kq {
 
+// gather the colour vector at site n of right-hand side j (base pointer src[j])
+template <class V2>
+__device__ __forceinline__ void gather(const V2 *src, uint32_t n, uint32_t Vh, V2 v[3])
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        v[c] = src[(size_t)c * Vh + n];
+}
+
 template <class A, class LK, int NR, int MU, int H>
-__device__ __forceinline__ void hop_fwd(const A &a, const LK &lk, uint32_t s, const SiteCoord &sc, const int *rr,
-                                        typename A::R acc[NR][6])
+__device__ __forceinline__ void hop_fwd(const A &a, const LK &lk, uint32_t s, const SiteCoord &sc,
+                                        const typename A::V2 *const src[NR], typename A::R acc[NR][6])
 {
     using R = typename A::R;
     c2_t<R> U[9];
@@ -23,14 +32,14 @@ __device__ __forceinline__ void hop_fwd(const A &a, const LK &lk, uint32_t s, co
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
         typename A::V2 v[3];
-        a.src.load(rr[j], n, v);
+        gather(src[j], n, a.g.Vh, v);
         umv_add(U, v, acc[j]);
     }
 }
 
 template <class A, class LK, int NR, int MU, int H>
-__device__ __forceinline__ void hop_bwd(const A &a, const LK &lk, uint32_t s, const SiteCoord &sc, const int *rr,
-                                        typename A::R acc[NR][6])
+__device__ __forceinline__ void hop_bwd(const A &a, const LK &lk, uint32_t s, const SiteCoord &sc,
+                                        const typename A::V2 *const src[NR], typename A::R acc[NR][6])
 {
     using R = typename A::R;
     const uint32_t n = nb_index<MU>(a.g, s, sc, -H);
@@ -39,7 +48,7 @@ __device__ __forceinline__ void hop_bwd(const A &a, const LK &lk, uint32_t s, co
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
         typename A::V2 v[3];
-        a.src.load(rr[j], n, v);
+        gather(src[j], n, a.g.Vh, v);
         udmv_sub(U, v, acc[j]);
     }
 }
@@ -58,11 +67,19 @@ __global__ void __launch_bounds__(NT) k_dslash(const DslashArgs<LF, LL, SR> a)
     if (j0 >= nact)
         return;  // uniform over the block
     int rr[NR];
+    // per right-hand-side base pointers, resolved once (uniform over the block)
+    const V2 *src[NR];
+    V2 *dst[NR];
+    const V2 *aux[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
         const int jj = (j0 + j < nact) ? j0 + j : nact - 1;  // duplicate the last one as padding
         rr[j] = a.rlist ? a.rlist[jj] : jj;
+        src[j] = a.src.p + a.src.base(rr[j]);
+        dst[j] = a.dst.p + a.dst.base(rr[j]);
+        aux[j] = EPI == EPI_AXMY_DOT ? a.aux.p + a.aux.base(rr[j]) : nullptr;
     }
+    const uint32_t Vh = a.g.Vh;
     const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t s0 = blk * NT + threadIdx.x;
     const bool valid = s0 < a.g.Vh;
@@ -75,34 +92,32 @@ __global__ void __launch_bounds__(NT) k_dslash(const DslashArgs<LF, LL, SR> a)
             acc[j][k] = 0;
     if (valid) {
         const SiteCoord sc = site_coords(a.g, s, a.dst_parity);
-        hop_fwd<A, LF, NR, 0, 1>(a, a.fat, s, sc, rr, acc);
-        hop_fwd<A, LL, NR, 0, 3>(a, a.lng, s, sc, rr, acc);
-        hop_fwd<A, LF, NR, 1, 1>(a, a.fat, s, sc, rr, acc);
-        hop_fwd<A, LL, NR, 1, 3>(a, a.lng, s, sc, rr, acc);
-        hop_fwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, rr, acc);
-        hop_fwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, rr, acc);
-        hop_fwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, rr, acc);
-        hop_fwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, rr, acc);
-        hop_bwd<A, LF, NR, 0, 1>(a, a.fat, s, sc, rr, acc);
-        hop_bwd<A, LL, NR, 0, 3>(a, a.lng, s, sc, rr, acc);
-        hop_bwd<A, LF, NR, 1, 1>(a, a.fat, s, sc, rr, acc);
-        hop_bwd<A, LL, NR, 1, 3>(a, a.lng, s, sc, rr, acc);
-        hop_bwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, rr, acc);
-        hop_bwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, rr, acc);
-        hop_bwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, rr, acc);
-        hop_bwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, rr, acc);
+        hop_fwd<A, LF, NR, 0, 1>(a, a.fat, s, sc, src, acc);
+        hop_fwd<A, LL, NR, 0, 3>(a, a.lng, s, sc, src, acc);
+        hop_fwd<A, LF, NR, 1, 1>(a, a.fat, s, sc, src, acc);
+        hop_fwd<A, LL, NR, 1, 3>(a, a.lng, s, sc, src, acc);
+        hop_fwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
+        hop_fwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
+        hop_fwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
+        hop_fwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
+        hop_bwd<A, LF, NR, 0, 1>(a, a.fat, s, sc, src, acc);
+        hop_bwd<A, LL, NR, 0, 3>(a, a.lng, s, sc, src, acc);
+        hop_bwd<A, LF, NR, 1, 1>(a, a.fat, s, sc, src, acc);
+        hop_bwd<A, LL, NR, 1, 3>(a, a.lng, s, sc, src, acc);
+        hop_bwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
+        hop_bwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
+        hop_bwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
+        hop_bwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
     }
     if constexpr (EPI == EPI_STORE) {
         if (valid) {
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
-                V2 o[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    o[c] = mk<V2>((SR)acc[j][2 * c], (SR)acc[j][2 * c + 1]);
                 // padding slots repeat the last live right-hand side, so the
                 // store is unconditional (same value to the same address)
-                a.dst.store(rr[j], s, o);
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    dst[j][(size_t)c * Vh + s] = mk<V2>((SR)acc[j][2 * c], (SR)acc[j][2 * c + 1]);
             }
         }
     } else {
@@ -112,16 +127,14 @@ __global__ void __launch_bounds__(NT) k_dslash(const DslashArgs<LF, LL, SR> a)
             double d = 0.0;
             if (valid && live) {
                 const double sg = a.sigma[rr[j]];
-                V2 p[3], o[3];
-                a.aux.load(rr[j], s, p);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const SR ox = (SR)((R)sg * (R)p[c].x - acc[j][2 * c]);
-                    const SR oy = (SR)((R)sg * (R)p[c].y - acc[j][2 * c + 1]);
-                    o[c] = mk<V2>(ox, oy);
-                    d += (double)p[c].x * (double)ox + (double)p[c].y * (double)oy;
+                    const V2 p = aux[j][(size_t)c * Vh + s];
+                    const SR ox = (SR)((R)sg * (R)p.x - acc[j][2 * c]);
+                    const SR oy = (SR)((R)sg * (R)p.y - acc[j][2 * c + 1]);
+                    dst[j][(size_t)c * Vh + s] = mk<V2>(ox, oy);
+                    d += (double)p.x * (double)ox + (double)p.y * (double)oy;
                 }
-                a.dst.store(rr[j], s, o);
             }
             const double tot = block_sum<NT>(d, lds);
             if (threadIdx.x == 0 && live)

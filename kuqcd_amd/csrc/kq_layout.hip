@@ -209,11 +209,19 @@ void convert_copy(const c2_t<XR> *x, c2_t<YR> *y, size_t n, hipStream_t st)
     KQ_HIP_CHECK(hipGetLastError());
 }
 
-__global__ void k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, size_t n)
+// four 16-byte elements per thread, each wavefront-coalesced
+__global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, size_t n)
 {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        b[i] = a[i];
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + 256 * k < n)
+            v[k] = a[base + 256 * k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + 256 * k < n)
+            b[base + 256 * k] = v[k];
 }
 
 void device_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
@@ -221,7 +229,7 @@ void device_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
     const size_t n = bytes / 16;
     if (n == 0)
         return;
-    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    const unsigned nb = (unsigned)((n + 1023) / 1024);
     hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, st, static_cast<const uint4 *>(src), static_cast<uint4 *>(dst),
                        n);
     KQ_HIP_CHECK(hipGetLastError());
