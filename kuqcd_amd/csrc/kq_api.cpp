@@ -78,12 +78,12 @@ Geom slot_geom(int slot)
 LinkD link_d(const GaugeSlot &s)
 {
     const uint32_t Vh = (uint32_t)((uint64_t)s.lat[0] * s.lat[1] * s.lat[2] * s.lat[3] / 2);
-    return LinkD{s.soa.as<double2>(), Vh};
+    return LinkD{s.kf.as<char>(), nb64_of(Vh)};
 }
 LinkF link_f(const GaugeSlot &s)
 {
     const uint32_t Vh = (uint32_t)((uint64_t)s.lat[0] * s.lat[1] * s.lat[2] * s.lat[3] / 2);
-    return LinkF{s.soa.as<float2>(), Vh};
+    return LinkF{s.kf.as<char>(), nb64_of(Vh)};
 }
 
 DerivedFormats &derived(int slot, bool want_f32, bool want_f16)
@@ -94,32 +94,29 @@ DerivedFormats &derived(int slot, bool want_f32, bool want_f16)
     if (f.prec != KUQCD_DOUBLE || l.prec != KUQCD_DOUBLE)
         throw std::invalid_argument("lower-precision formats are derived from a double-precision gauge");
     const uint32_t Vh = slot_geom(slot).Vh;
-    const size_t ncomp = (size_t)8 * 9 * Vh;
     if (want_f32 && !d.f32) {
-        d.fat_f.alloc(sizeof(float2) * ncomp);
-        d.lng_f.alloc(sizeof(float2) * ncomp);
-        soa_d_to_f(f.soa.as<double2>(), d.fat_f.as<float2>(), ncomp, c.stream);
-        soa_d_to_f(l.soa.as<double2>(), d.lng_f.as<float2>(), ncomp, c.stream);
+        d.fat_f.alloc(link_field_bytes<LinkF>(Vh));
+        d.lng_f.alloc(link_field_bytes<LinkF>(Vh));
+        canon_to_linkF<double>(f.soa.as<double2>(), d.fat_f.as<char>(), Vh, c.stream);
+        canon_to_linkF<double>(l.soa.as<double2>(), d.lng_f.as<char>(), Vh, c.stream);
         KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
         d.f32 = true;
     }
     if (want_f16 && !d.f16) {
-        d.fat_h.alloc(sizeof(__half2) * ncomp);
-        d.fat_hs.alloc(sizeof(float) * 8 * (size_t)Vh);
-        soa_d_to_h(f.soa.as<double2>(), d.fat_h.as<__half2>(), d.fat_hs.as<float>(), Vh, c.stream);
+        d.fat_h.alloc(link_field_bytes<LinkH>(Vh));
+        canon_to_linkH(f.soa.as<double2>(), d.fat_h.as<char>(), Vh, c.stream);
         // long links: try 12-real storage, fall back to 18 reals
         DevBuf fail(sizeof(int));
         KQ_HIP_CHECK(hipMemsetAsync(fail.as<void>(), 0, sizeof(int), c.stream));
-        d.lng_h.alloc(sizeof(__half2) * (size_t)8 * 6 * Vh);
-        d.lng_hs.alloc(sizeof(float) * 8 * (size_t)Vh);
-        soa_d_to_h12(l.soa.as<double2>(), d.lng_h.as<__half2>(), d.lng_hs.as<float>(), Vh, fail.as<int>(), c.stream);
+        d.lng_h.alloc(link_field_bytes<LinkH12>(Vh));
+        canon_to_linkH12(l.soa.as<double2>(), d.lng_h.as<char>(), Vh, fail.as<int>(), c.stream);
         int nfail = 0;
         KQ_HIP_CHECK(hipMemcpyAsync(&nfail, fail.as<void>(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
         KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
         d.long12 = (nfail == 0);
         if (!d.long12) {
-            d.lng_h.alloc(sizeof(__half2) * ncomp);
-            soa_d_to_h(l.soa.as<double2>(), d.lng_h.as<__half2>(), d.lng_hs.as<float>(), Vh, c.stream);
+            d.lng_h.alloc(link_field_bytes<LinkH>(Vh));
+            canon_to_linkH(l.soa.as<double2>(), d.lng_h.as<char>(), Vh, c.stream);
             KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
         }
         d.f16 = true;
@@ -212,6 +209,15 @@ static void upload_links(const HR *h, GaugeSlot &slot, double scale, int prec)
             gauge_host_to_soa<HR, float>(stage.as<HR>(), i0, n, slot.soa.as<float2>(), Vh, scale, c.stream);
         KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
     }
+    // the kernel format of the loaded precision
+    if (prec == KUQCD_DOUBLE) {
+        slot.kf.alloc(link_field_bytes<LinkD>(Vh));
+        canon_to_linkD(slot.soa.as<double2>(), slot.kf.as<char>(), Vh, c.stream);
+    } else {
+        slot.kf.alloc(link_field_bytes<LinkF>(Vh));
+        canon_to_linkF<float>(slot.soa.as<float2>(), slot.kf.as<char>(), Vh, c.stream);
+    }
+    KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
 }
 
 static KuQCDStatus load_gauge(void *h_gauge, KuQCDGaugeParam *param)
@@ -284,6 +290,7 @@ static KuQCDStatus load_gauge(void *h_gauge, KuQCDGaugeParam *param)
     GaugeSlot &gs = (param->link_type == KUQCD_FAT_LINK) ? c.fat[slot] : c.lng[slot];
     gs.loaded = false;
     gs.soa.release();
+    gs.kf.release();
     free_derived(slot);
     for (int i = 0; i < 4; ++i)
         gs.lat[i] = param->lat[i];

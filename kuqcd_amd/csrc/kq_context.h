@@ -13,21 +13,28 @@
 namespace kq {
 
 // A loaded link field (fat or long) in the precision it was loaded in,
-// pre-scaled (1/2 fat, -1/48 long) in SoA layout.
+// pre-scaled (1/2 fat, -1/48 long): the canonical SoA copy
+// canon[((q*4 + mu)*9 + k)*Vh + s] from which every kernel format is made,
+// and the kernel format of the loaded precision (LinkD or LinkF).
 struct GaugeSlot {
     bool loaded = false;
     int prec = 0;  // 4 or 8
     unsigned lat[4] = {0, 0, 0, 0};
-    DevBuf soa;  // double2 or float2, 4 dirs x 2 parities x 9 complex x Vh
+    DevBuf soa;  // canonical, double2 or float2
+    DevBuf kf;   // LinkD (double) or LinkF (float) format
 };
 
-// Lower-precision copies of a double gauge pair, made on first use.
+// Lower-precision kernel formats of a double gauge pair, made on first use.
 struct DerivedFormats {
     bool f32 = false, f16 = false;
-    bool long12 = false;  // long links stored as 12 reals
-    DevBuf fat_f, lng_f;
-    DevBuf fat_h, fat_hs, lng_h, lng_hs;
+    bool long12 = false;  // long links stored as 12 reals (LinkH12), else LinkH
+    DevBuf fat_f, lng_f;  // LinkF
+    DevBuf fat_h, lng_h;  // LinkH / LinkH12
 };
+
+inline LinkF as_linkF(const DevBuf &b, uint32_t Vh) { return LinkF{b.as<char>(), nb64_of(Vh)}; }
+inline LinkH as_linkH(const DevBuf &b, uint32_t Vh) { return LinkH{b.as<char>(), nb64_of(Vh)}; }
+inline LinkH12 as_linkH12(const DevBuf &b, uint32_t Vh) { return LinkH12{b.as<char>(), nb64_of(Vh)}; }
 
 struct Context {
     bool device_ready = false;

@@ -1,12 +1,11 @@
 // Device storage formats for gauge links and colour vectors on the even-odd
 // checkerboard, with the accessors the kernels use.
 //
-// HBM layout (structure of arrays, site index fastest so that a wavefront's
-// 64 lanes read 64 consecutive sites with one coalesced load per component):
-//   links   : comp[((q*4 + mu)*NC + k)*Vh + s]   q = parity block, NC components
-//   scales  : scale[(q*4 + mu)*Vh + s]           (FP16 formats only)
-//   vectors : v[off(r) + c*Vh + s]               r = right-hand side, c = colour,
-//             off(r) = r*rstride or a per-right-hand-side offset table
+// HBM layout:
+//   links   : per (parity q, direction mu), blocks of 64 sites, each block an
+//             array of 16/8/4-byte chunks x 64 lanes (see link_block_offset)
+//   vectors : structure of arrays, v[off(r) + c*Vh + s], r = right-hand side,
+//             c = colour, off(r) = r*rstride or a per-right-hand-side offset table
 //
 // Device links are pre-scaled at load time: fat links by 1/2 and long links by
 // c3000 = -1/48 (dirac_hisq.cpp:40, 473, 480), so the hopping term is
@@ -21,70 +20,91 @@ namespace kq {
 // application, so it should not displace the reused colour vectors in L2.
 typedef double kq_v2d __attribute__((ext_vector_type(2)));
 typedef float kq_v2f __attribute__((ext_vector_type(2)));
-__device__ inline double2 ld_stream(const double2 *p)
+typedef float kq_v4f __attribute__((ext_vector_type(4)));
+typedef unsigned int kq_v4u __attribute__((ext_vector_type(4)));
+typedef unsigned int kq_v2u __attribute__((ext_vector_type(2)));
+__device__ inline kq_v2d ld_nt(const kq_v2d *p) { return __builtin_nontemporal_load(p); }
+__device__ inline kq_v4f ld_nt(const kq_v4f *p) { return __builtin_nontemporal_load(p); }
+__device__ inline kq_v2f ld_nt(const kq_v2f *p) { return __builtin_nontemporal_load(p); }
+__device__ inline kq_v4u ld_nt(const kq_v4u *p) { return __builtin_nontemporal_load(p); }
+__device__ inline kq_v2u ld_nt(const kq_v2u *p) { return __builtin_nontemporal_load(p); }
+__device__ inline float ld_nt(const float *p) { return __builtin_nontemporal_load(p); }
+
+__device__ inline float2 h2f(unsigned int u)
 {
-    kq_v2d v = __builtin_nontemporal_load(reinterpret_cast<const kq_v2d *>(p));
-    return mk<double2>(v.x, v.y);
-}
-__device__ inline float2 ld_stream(const float2 *p)
-{
-    kq_v2f v = __builtin_nontemporal_load(reinterpret_cast<const kq_v2f *>(p));
-    return mk<float2>(v.x, v.y);
-}
-__device__ inline __half2 ld_stream(const __half2 *p)
-{
-    unsigned int u = __builtin_nontemporal_load(reinterpret_cast<const unsigned int *>(p));
-    return *reinterpret_cast<__half2 *>(&u);
-}
-__device__ inline float ld_stream(const float *p)
-{
-    return __builtin_nontemporal_load(p);
+    return __half22float2(*reinterpret_cast<const __half2 *>(&u));
 }
 
 // ------------------------------------------------------------ links
-struct LinkD {  // FP64, 18 reals
+// Link fields are stored per (parity q, direction mu) in blocks of 64 sites
+// (one wavefront): block b of (q, mu) starts at byte ((q*4 + mu)*nb64 + b) *
+// kBlock and holds, for each 16-, 8- or 4-byte chunk of a link, the 64 lanes'
+// chunks back to back. A wavefront's load of one chunk is one contiguous,
+// aligned 1 KiB / 512 B / 256 B segment, and the whole link of 64 sites is one
+// contiguous region of 64 * kBytes.
+__host__ __device__ inline size_t link_block_offset(int q, int mu, uint32_t s, uint32_t nb64, size_t block_bytes)
+{
+    return ((size_t)(q * 4 + mu) * nb64 + (s >> 6)) * block_bytes;
+}
+
+struct LinkD {  // FP64, 18 reals: 9 chunks of 16 B (one complex each)
     using R = double;
     static constexpr int kBytes = 144;
-    const double2 *p;
-    uint32_t Vh;
+    static constexpr size_t kBlock = 64 * kBytes;
+    const char *p;
+    uint32_t nb64;
     __device__ inline void load(int q, int mu, uint32_t s, double2 u[9]) const
     {
-        const double2 *b = p + (size_t)((q * 4 + mu) * 9) * Vh + s;
+        const kq_v2d *b = reinterpret_cast<const kq_v2d *>(p + link_block_offset(q, mu, s, nb64, kBlock)) + (s & 63);
 #pragma unroll
-        for (int k = 0; k < 9; ++k)
-            u[k] = ld_stream(b + (size_t)k * Vh);
+        for (int k = 0; k < 9; ++k) {
+            const kq_v2d v = ld_nt(b + 64 * k);
+            u[k] = mk<double2>(v.x, v.y);
+        }
     }
 };
 
-struct LinkF {  // FP32, 18 reals
+struct LinkF {  // FP32, 18 reals: 4 chunks of 16 B (complex 0..7) + 1 of 8 B (complex 8)
     using R = float;
     static constexpr int kBytes = 72;
-    const float2 *p;
-    uint32_t Vh;
+    static constexpr size_t kBlock = 64 * kBytes;
+    const char *p;
+    uint32_t nb64;
     __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
     {
-        const float2 *b = p + (size_t)((q * 4 + mu) * 9) * Vh + s;
+        const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
+        const uint32_t l = s & 63;
 #pragma unroll
-        for (int k = 0; k < 9; ++k)
-            u[k] = ld_stream(b + (size_t)k * Vh);
+        for (int c = 0; c < 4; ++c) {
+            const kq_v4f v = ld_nt(reinterpret_cast<const kq_v4f *>(b + 1024 * c) + l);
+            u[2 * c] = mk<float2>(v.x, v.y);
+            u[2 * c + 1] = mk<float2>(v.z, v.w);
+        }
+        const kq_v2f w = ld_nt(reinterpret_cast<const kq_v2f *>(b + 4096) + l);
+        u[8] = mk<float2>(w.x, w.y);
     }
 };
 
 // FP16, 18 reals, one FP32 scale per link: U = scale * half(U/scale).
+// Chunks: 16 B (complex 0..3), 16 B (complex 4..7), 8 B (complex 8 + scale).
 struct LinkH {
     using R = float;
     static constexpr int kBytes = 40;
-    const __half2 *p;
-    const float *scale;
-    uint32_t Vh;
+    static constexpr size_t kBlock = 64 * kBytes;
+    const char *p;
+    uint32_t nb64;
     __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
     {
-        const __half2 *b = p + (size_t)((q * 4 + mu) * 9) * Vh + s;
-        const float a = ld_stream(scale + (size_t)(q * 4 + mu) * Vh + s);
+        const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
+        const uint32_t l = s & 63;
+        const kq_v4u c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
+        const kq_v4u c1 = ld_nt(reinterpret_cast<const kq_v4u *>(b + 1024) + l);
+        const kq_v2u c2 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 2048) + l);
+        const float a = __uint_as_float(c2.y);
+        const unsigned int h[9] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x};
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            __half2 h = ld_stream(b + (size_t)k * Vh);
-            float2 f = __half22float2(h);
+            const float2 f = h2f(h[k]);
             u[k] = mk<float2>(f.x * a, f.y * a);
         }
     }
@@ -92,26 +112,33 @@ struct LinkH {
 
 // FP16, 12 reals (rows 0 and 1 of an SU(3) matrix V), one signed FP32 scale b
 // per link: U = b * V with row 2 of V = conj(row0 x row1).
+// Chunks: 16 B (complex 0..3), 8 B (complex 4, 5), 4 B (scale).
 struct LinkH12 {
     using R = float;
     static constexpr int kBytes = 28;
-    const __half2 *p;
-    const float *scale;
-    uint32_t Vh;
+    static constexpr size_t kBlock = 64 * kBytes;
+    const char *p;
+    uint32_t nb64;
     __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
     {
-        const __half2 *b = p + (size_t)((q * 4 + mu) * 6) * Vh + s;
-        const float a = ld_stream(scale + (size_t)(q * 4 + mu) * Vh + s);
+        const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
+        const uint32_t l = s & 63;
+        const kq_v4u c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
+        const kq_v2u c1 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 1024) + l);
+        const float a = ld_nt(reinterpret_cast<const float *>(b + 1536) + l);
         float2 v[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-            v[k] = __half22float2(ld_stream(b + (size_t)k * Vh));
+        v[0] = h2f(c0.x);
+        v[1] = h2f(c0.y);
+        v[2] = h2f(c0.z);
+        v[3] = h2f(c0.w);
+        v[4] = h2f(c1.x);
+        v[5] = h2f(c1.y);
         // row2 = conj(row0 x row1)
         float2 w[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-            const float2 a1 = v[c1], b2 = v[3 + c2], a2 = v[c2], b1 = v[3 + c1];
+            const int c1i = (c + 1) % 3, c2i = (c + 2) % 3;
+            const float2 a1 = v[c1i], b2 = v[3 + c2i], a2 = v[c2i], b1 = v[3 + c1i];
             const float re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
             const float im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
             w[c] = mk<float2>(re, -im);
@@ -124,6 +151,10 @@ struct LinkH12 {
             u[6 + c] = mk<float2>(w[c].x * a, w[c].y * a);
     }
 };
+
+inline uint32_t nb64_of(uint32_t Vh) { return (Vh + 63) / 64; }
+// bytes of one link field (both parities, 4 directions) in format LK
+template <class LK> inline size_t link_field_bytes(uint32_t Vh) { return (size_t)8 * nb64_of(Vh) * LK::kBlock; }
 
 // ------------------------------------------------------------ colour vectors
 template <typename R> struct Vec {

@@ -17,15 +17,16 @@ template <typename HR, typename LR>
 void gauge_host_to_soa(const HR *raw, size_t i0, size_t n, c2_t<LR> *dst, uint32_t Vh, double scale,
                        hipStream_t st);
 
-// FP64 SoA links -> FP32 SoA links (rounding to nearest)
-void soa_d_to_f(const double2 *src, float2 *dst, size_t nlinks_x9, hipStream_t st);
-
-// FP64 SoA links -> FP16 with one FP32 scale per link (max |entry|)
-void soa_d_to_h(const double2 *src, __half2 *dst, float *scale, uint32_t Vh, hipStream_t st);
-
-// FP64 SoA links of the form b * V (V in SU(3), b real) -> rows 0,1 of V in
+// Kernel link formats (kq_fields.h) from the canonical SoA links
+// canon[((q*4 + mu)*9 + k)*Vh + s] (complex):
+// FP64 -> LinkD; FP64 or FP32 -> LinkF (rounding to nearest)
+void canon_to_linkD(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st);
+template <typename CR> void canon_to_linkF(const c2_t<CR> *canon, char *dst, uint32_t Vh, hipStream_t st);
+// FP64 -> LinkH: FP16 entries with one FP32 scale per link (max |entry|)
+void canon_to_linkH(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st);
+// FP64 links of the form b * V (V in SU(3), b real) -> LinkH12: rows 0,1 of V in
 // FP16 and b in FP32. fail[0] counts links that are not of that form to 1e-6.
-void soa_d_to_h12(const double2 *src, __half2 *dst, float *scale, uint32_t Vh, int *fail, hipStream_t st);
+void canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);
 
 // host-layout colour vector (device copy) <-> SoA vector of right-hand side r
 template <typename HR, typename VR>

@@ -38,47 +38,94 @@ void gauge_host_to_soa(const HR *raw, size_t i0, size_t n, c2_t<LR> *dst, uint32
     KQ_HIP_CHECK(hipGetLastError());
 }
 
-__global__ void k_soa_d_to_f(const double2 *src, float2 *dst, size_t n)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        dst[i] = mk<float2>((float)src[i].x, (float)src[i].y);
-}
-
-void soa_d_to_f(const double2 *src, float2 *dst, size_t n, hipStream_t st)
-{
-    if (n == 0)
-        return;
-    hipLaunchKernelGGL(k_soa_d_to_f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
-    KQ_HIP_CHECK(hipGetLastError());
-}
-
 // one thread per link (q, mu, s)
-__global__ void k_soa_d_to_h(const double2 *src, __half2 *dst, float *scale, uint32_t Vh)
+__global__ void k_canon_to_linkD(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)8 * Vh)
         return;
-    const size_t qm = t / Vh, s = t - qm * Vh;
-    double amax = 0.0;
-    for (int k = 0; k < 9; ++k) {
-        const double2 v = src[(qm * 9 + k) * Vh + s];
-        amax = fmax(amax, fmax(fabs(v.x), fabs(v.y)));
-    }
-    const float a = amax > 0.0 ? (float)amax : 1.0f;
-    for (int k = 0; k < 9; ++k) {
-        const double2 v = src[(qm * 9 + k) * Vh + s];
-        dst[(qm * 9 + k) * Vh + s] = __floats2half2_rn((float)(v.x / a), (float)(v.y / a));
-    }
-    scale[qm * Vh + s] = a;
+    const uint32_t qm = (uint32_t)(t / Vh), s = (uint32_t)(t - (size_t)qm * Vh);
+    double2 *b = reinterpret_cast<double2 *>(dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkD::kBlock)) + (s & 63);
+    for (int k = 0; k < 9; ++k)
+        b[64 * k] = canon[((size_t)qm * 9 + k) * Vh + s];
 }
 
-void soa_d_to_h(const double2 *src, __half2 *dst, float *scale, uint32_t Vh, hipStream_t st)
+void canon_to_linkD(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st)
 {
     const size_t n = (size_t)8 * Vh;
     if (n == 0)
         return;
-    hipLaunchKernelGGL(k_soa_d_to_h, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, scale, Vh);
+    hipLaunchKernelGGL(k_canon_to_linkD, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
+                       nb64_of(Vh));
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+template <typename CR>
+__global__ void k_canon_to_linkF(const c2_t<CR> *canon, char *dst, uint32_t Vh, uint32_t nb64)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)8 * Vh)
+        return;
+    const uint32_t qm = (uint32_t)(t / Vh), s = (uint32_t)(t - (size_t)qm * Vh);
+    char *b = dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkF::kBlock);
+    const uint32_t l = s & 63;
+    float v[18];
+    for (int k = 0; k < 9; ++k) {
+        const c2_t<CR> c = canon[((size_t)qm * 9 + k) * Vh + s];
+        v[2 * k] = (float)c.x;
+        v[2 * k + 1] = (float)c.y;
+    }
+    for (int c = 0; c < 4; ++c)
+        reinterpret_cast<float4 *>(b + 1024 * c)[l] = make_float4(v[4 * c], v[4 * c + 1], v[4 * c + 2], v[4 * c + 3]);
+    reinterpret_cast<float2 *>(b + 4096)[l] = make_float2(v[16], v[17]);
+}
+
+template <typename CR> void canon_to_linkF(const c2_t<CR> *canon, char *dst, uint32_t Vh, hipStream_t st)
+{
+    const size_t n = (size_t)8 * Vh;
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL((k_canon_to_linkF<CR>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
+                       nb64_of(Vh));
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+__device__ inline unsigned int pack_h2(float a, float b)
+{
+    __half2 h = __floats2half2_rn(a, b);
+    return *reinterpret_cast<unsigned int *>(&h);
+}
+
+__global__ void k_canon_to_linkH(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)8 * Vh)
+        return;
+    const uint32_t qm = (uint32_t)(t / Vh), s = (uint32_t)(t - (size_t)qm * Vh);
+    double2 m[9];
+    double amax = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        m[k] = canon[((size_t)qm * 9 + k) * Vh + s];
+        amax = fmax(amax, fmax(fabs(m[k].x), fabs(m[k].y)));
+    }
+    const float a = amax > 0.0 ? (float)amax : 1.0f;
+    unsigned int h[9];
+    for (int k = 0; k < 9; ++k)
+        h[k] = pack_h2((float)(m[k].x / a), (float)(m[k].y / a));
+    char *b = dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkH::kBlock);
+    const uint32_t l = s & 63;
+    reinterpret_cast<uint4 *>(b)[l] = make_uint4(h[0], h[1], h[2], h[3]);
+    reinterpret_cast<uint4 *>(b + 1024)[l] = make_uint4(h[4], h[5], h[6], h[7]);
+    reinterpret_cast<uint2 *>(b + 2048)[l] = make_uint2(h[8], __float_as_uint(a));
+}
+
+void canon_to_linkH(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st)
+{
+    const size_t n = (size_t)8 * Vh;
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_canon_to_linkH, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
+                       nb64_of(Vh));
     KQ_HIP_CHECK(hipGetLastError());
 }
 
@@ -94,22 +141,22 @@ __device__ inline double2 det3(const double2 m[9])
     return add(add(mul(m[0], c0), mul(m[1], c1)), mul(m[2], c2));
 }
 
-__global__ void k_soa_d_to_h12(const double2 *src, __half2 *dst, float *scale, uint32_t Vh, int *fail)
+__global__ void k_canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64, int *fail)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)8 * Vh)
         return;
-    const size_t qm = t / Vh, s = t - qm * Vh;
+    const uint32_t qm = (uint32_t)(t / Vh), s = (uint32_t)(t - (size_t)qm * Vh);
     double2 m[9];
     for (int k = 0; k < 9; ++k)
-        m[k] = src[(qm * 9 + k) * Vh + s];
+        m[k] = canon[((size_t)qm * 9 + k) * Vh + s];
     const double2 d = det3(m);
     // b^3 = det(b V) must be real; b = cbrt(Re det)
-    const double b = cbrt(d.x);
-    bool ok = fabs(d.y) <= 1e-6 * fabs(d.x) && b != 0.0;
+    const double bsc = cbrt(d.x);
+    bool ok = fabs(d.y) <= 1e-6 * fabs(d.x) && bsc != 0.0;
     double2 v[9];
     for (int k = 0; k < 9; ++k)
-        v[k] = ok ? mk<double2>(m[k].x / b, m[k].y / b) : mk<double2>(0.0, 0.0);
+        v[k] = ok ? mk<double2>(m[k].x / bsc, m[k].y / bsc) : mk<double2>(0.0, 0.0);
     if (ok) {
         // unitarity of V: rows orthonormal
         for (int r1 = 0; r1 < 3 && ok; ++r1)
@@ -127,18 +174,23 @@ __global__ void k_soa_d_to_h12(const double2 *src, __half2 *dst, float *scale, u
     }
     if (!ok)
         atomicAdd(fail, 1);
+    unsigned int h[6];
     for (int k = 0; k < 6; ++k)
-        dst[(qm * 6 + k) * Vh + s] = __floats2half2_rn((float)v[k].x, (float)v[k].y);
-    scale[qm * Vh + s] = (float)b;
+        h[k] = pack_h2((float)v[k].x, (float)v[k].y);
+    char *b = dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkH12::kBlock);
+    const uint32_t l = s & 63;
+    reinterpret_cast<uint4 *>(b)[l] = make_uint4(h[0], h[1], h[2], h[3]);
+    reinterpret_cast<uint2 *>(b + 1024)[l] = make_uint2(h[4], h[5]);
+    reinterpret_cast<float *>(b + 1536)[l] = (float)bsc;
 }
 
-void soa_d_to_h12(const double2 *src, __half2 *dst, float *scale, uint32_t Vh, int *fail, hipStream_t st)
+void canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
 {
     const size_t n = (size_t)8 * Vh;
     if (n == 0)
         return;
-    hipLaunchKernelGGL(k_soa_d_to_h12, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, scale, Vh,
-                       fail);
+    hipLaunchKernelGGL(k_canon_to_linkH12, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
+                       nb64_of(Vh), fail);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
@@ -240,6 +292,8 @@ template void gauge_host_to_soa<double, double>(const double *, size_t, size_t, 
 template void gauge_host_to_soa<float, float>(const float *, size_t, size_t, float2 *, uint32_t, double, hipStream_t);
 template void gauge_host_to_soa<float, double>(const float *, size_t, size_t, double2 *, uint32_t, double,
                                                hipStream_t);
+template void canon_to_linkF<double>(const double2 *, char *, uint32_t, hipStream_t);
+template void canon_to_linkF<float>(const float2 *, char *, uint32_t, hipStream_t);
 template void vec_host_to_soa<double, double>(const double *, Vec<double>, int, hipStream_t);
 template void vec_host_to_soa<float, float>(const float *, Vec<float>, int, hipStream_t);
 template void vec_host_to_soa<double, float>(const double *, Vec<float>, int, hipStream_t);

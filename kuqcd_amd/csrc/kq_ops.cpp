@@ -110,18 +110,18 @@ static KuQCDStatus apply(int nrhs, const void *src, void *dst, int parity, int m
     }
     if (mode == KQ_MODE_MIXED_FP32) {
         DerivedFormats &d = derived(slot, true, false);
-        apply_t<LinkF, LinkF, float, double>(LinkF{d.fat_f.as<float2>(), g.Vh}, LinkF{d.lng_f.as<float2>(), g.Vh}, g,
+        apply_t<LinkF, LinkF, float, double>(as_linkF(d.fat_f, g.Vh), as_linkF(d.lng_f, g.Vh), g,
                                              nrhs, src, dst, parity, nr, nt, mdaggm, sigma);
         return KUQCD_SUCCESS;
     }
     if (mode == KQ_MODE_MIXED_FP16) {
         DerivedFormats &d = derived(slot, false, true);
-        LinkH fh{d.fat_h.as<__half2>(), d.fat_hs.as<float>(), g.Vh};
+        LinkH fh = as_linkH(d.fat_h, g.Vh);
         if (d.long12)
-            apply_t<LinkH, LinkH12, float, double>(fh, LinkH12{d.lng_h.as<__half2>(), d.lng_hs.as<float>(), g.Vh}, g,
+            apply_t<LinkH, LinkH12, float, double>(fh, as_linkH12(d.lng_h, g.Vh), g,
                                                    nrhs, src, dst, parity, nr, nt, mdaggm, sigma);
         else
-            apply_t<LinkH, LinkH, float, double>(fh, LinkH{d.lng_h.as<__half2>(), d.lng_hs.as<float>(), g.Vh}, g, nrhs,
+            apply_t<LinkH, LinkH, float, double>(fh, as_linkH(d.lng_h, g.Vh), g, nrhs,
                                                  src, dst, parity, nr, nt, mdaggm, sigma);
         return KUQCD_SUCCESS;
     }
@@ -197,16 +197,16 @@ static double time_mode(int slot, int mode, int nrhs, int nr, int nt, int iters,
     }
     if (mode == KQ_MODE_MIXED_FP32) {
         DerivedFormats &d = derived(slot, true, false);
-        return time_dslash<LinkF, LinkF, float>(LinkF{d.fat_f.as<float2>(), g.Vh}, LinkF{d.lng_f.as<float2>(), g.Vh},
+        return time_dslash<LinkF, LinkF, float>(as_linkF(d.fat_f, g.Vh), as_linkF(d.lng_f, g.Vh),
                                                 g, nrhs, nr, nt, iters);
     }
     DerivedFormats &d = derived(slot, false, true);
     *long12 = d.long12;
-    LinkH fh{d.fat_h.as<__half2>(), d.fat_hs.as<float>(), g.Vh};
+    LinkH fh = as_linkH(d.fat_h, g.Vh);
     if (d.long12)
-        return time_dslash<LinkH, LinkH12, float>(fh, LinkH12{d.lng_h.as<__half2>(), d.lng_hs.as<float>(), g.Vh}, g,
+        return time_dslash<LinkH, LinkH12, float>(fh, as_linkH12(d.lng_h, g.Vh), g,
                                                   nrhs, nr, nt, iters);
-    return time_dslash<LinkH, LinkH, float>(fh, LinkH{d.lng_h.as<__half2>(), d.lng_hs.as<float>(), g.Vh}, g, nrhs, nr,
+    return time_dslash<LinkH, LinkH, float>(fh, as_linkH(d.lng_h, g.Vh), g, nrhs, nr,
                                             nt, iters);
 }
 

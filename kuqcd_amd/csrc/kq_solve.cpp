@@ -159,6 +159,16 @@ static bool debug_on()
     return on;
 }
 
+// autotuned right-hand-side blocking for 0..min(max_sys, 16) live systems
+static std::vector<int> nr_table(int slot, int mode, int max_sys)
+{
+    const int n = std::min(max_sys, 16);
+    std::vector<int> t(n + 1, 1);
+    for (int k = 2; k <= n; ++k)
+        t[k] = tuned_launch(slot, mode, k).first;
+    return t;
+}
+
 static std::vector<double> abs_shifts(const KuQCDInvertParam &p, int r)
 {
     // Q2: system k >= 1 is K + shifts[0] + shifts[k] (inverter_cgm.cpp:52)
@@ -189,7 +199,10 @@ static void refine_pairs(Problem &P, LF fat, LL lng, DevBuf &x_out, const std::v
     Context &c = ctx();
     const size_t n3 = (size_t)3 * P.g.Vh;
     const int nsys = (int)need.size();
-    tune.nr = P.o->nr > 0 ? P.o->nr : tuned_launch(P.slot, P.o->mode, nsys).first;
+    if (P.o->nr > 0)
+        tune.nr = P.o->nr;
+    else
+        tune.nr_by_live = nr_table(P.slot, P.o->mode, nsys);
     CgmRunner<double, float, LF, LL> ref(P.g, P.parity, fat, lng, nsys, nsys, c.stream, tune);
     ref.enable_reliable(link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.delta);
     for (int i = 0; i < nsys; ++i) {
@@ -230,8 +243,11 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
                         : P.o->fork_tol > 0 ? P.o->fork_tol
                                             : (P.o->mode == KQ_MODE_MIXED_FP16 ? 1e-3 : 1e-5);
     const int max_sys = nrhs * ns;
-    // nr for the peak number of live systems (all multishift systems plus forks)
-    tune.nr = P.o->nr > 0 ? P.o->nr : tuned_launch(P.slot, P.o->mode, std::min(max_sys, 16)).first;
+    // right-hand-side blocking of the hopping term by number of live systems
+    if (P.o->nr > 0)
+        tune.nr = P.o->nr;
+    else
+        tune.nr_by_live = nr_table(P.slot, P.o->mode, max_sys);
     x_out.alloc(sizeof(double2) * nrhs * ns * n3);
 
     auto t0 = clk::now();
@@ -331,10 +347,12 @@ static void run_fp64(Problem &P, DevBuf &x_out, KqSolveReport *rep, std::vector<
 {
     Context &c = ctx();
     const size_t n3 = (size_t)3 * P.g.Vh;
-    const auto nrnt = tuned_launch(P.slot, KQ_MODE_FP64, P.nrhs);
     RunnerTuning tune;
-    tune.nr = P.o->nr > 0 ? P.o->nr : nrnt.first;
-    tune.nt = P.o->nt > 0 ? P.o->nt : nrnt.second;
+    tune.nt = P.o->nt > 0 ? P.o->nt : tuned_launch(P.slot, KQ_MODE_FP64, 1).second;
+    if (P.o->nr > 0)
+        tune.nr = P.o->nr;
+    else
+        tune.nr_by_live = nr_table(P.slot, KQ_MODE_FP64, P.nrhs);
     tune.chunk = P.o->chunk > 0 ? P.o->chunk : 16;
     auto t0 = clk::now();
     CgmRunner<double, double, LinkD, LinkD> run(P.g, P.parity, link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.nrhs,
@@ -442,16 +460,16 @@ KuQCDStatus solve_extended(int slot, int nrhs, void **x, void **b, KuQCDInvertPa
         run_fp64(P, xdev, rep, res2);
     } else if (mode == KQ_MODE_MIXED_FP32) {
         DerivedFormats &d = derived(slot, true, false);
-        run_mixed<LinkF, LinkF>(P, LinkF{d.fat_f.as<float2>(), P.g.Vh}, LinkF{d.lng_f.as<float2>(), P.g.Vh}, xdev,
+        run_mixed<LinkF, LinkF>(P, as_linkF(d.fat_f, P.g.Vh), as_linkF(d.lng_f, P.g.Vh), xdev,
                                 rep, res2);
     } else if (mode == KQ_MODE_MIXED_FP16) {
         DerivedFormats &d = derived(slot, false, true);
-        LinkH fh{d.fat_h.as<__half2>(), d.fat_hs.as<float>(), P.g.Vh};
+        LinkH fh = as_linkH(d.fat_h, P.g.Vh);
         if (d.long12)
-            run_mixed<LinkH, LinkH12>(P, fh, LinkH12{d.lng_h.as<__half2>(), d.lng_hs.as<float>(), P.g.Vh}, xdev, rep,
+            run_mixed<LinkH, LinkH12>(P, fh, as_linkH12(d.lng_h, P.g.Vh), xdev, rep,
                                       res2);
         else
-            run_mixed<LinkH, LinkH>(P, fh, LinkH{d.lng_h.as<__half2>(), d.lng_hs.as<float>(), P.g.Vh}, xdev, rep,
+            run_mixed<LinkH, LinkH>(P, fh, as_linkH(d.lng_h, P.g.Vh), xdev, rep,
                                     res2);
     } else {
         set_error("unknown mode");

@@ -117,6 +117,14 @@ struct RunnerTuning {
     int nr = 1;      // right-hand sides per thread in the batched hopping term
     int nt = 256;    // threads per block of the hopping term
     int chunk = 8;   // iterations replayed between host checks
+    // nr by number of live systems (index clamped to the last entry); empty: always nr
+    std::vector<int> nr_by_live;
+    int pick_nr(int live) const
+    {
+        if (nr_by_live.empty())
+            return nr;
+        return nr_by_live[std::min<size_t>((size_t)std::max(live, 0), nr_by_live.size() - 1)];
+    }
 };
 
 // Batched multishift CG over up to max_sys systems that share one gauge field.
@@ -162,8 +170,9 @@ public:
     }
     ~CgmRunner()
     {
-        if (graph_exec_)
-            (void)hipGraphExecDestroy(graph_exec_);
+        for (auto &g : graphs_)
+            if (g)
+                (void)hipGraphExecDestroy(g);
     }
     CgmRunner(const CgmRunner &) = delete;
     CgmRunner &operator=(const CgmRunner &) = delete;
@@ -237,12 +246,15 @@ public:
     void run(const DropFn &on_drop = nullptr)
     {
         start_pending();
-        if (!graph_exec_)
-            capture();
         static const bool dbg = getenv("KQ_DEBUG") != nullptr;
         for (;;) {
+            // one graph per right-hand-side blocking, chosen from the live count
+            int nlive = 0;
+            for (int sys = 0; sys < nsys_; ++sys)
+                nlive += (!state_h_[sys].stop && !state_h_[sys].pause) ? 1 : 0;
+            hipGraphExec_t gx = graph_for(tune_.pick_nr(nlive));
             for (int c = 0; c < tune_.chunk; ++c)
-                KQ_HIP_CHECK(hipGraphLaunch(graph_exec_, st_));
+                KQ_HIP_CHECK(hipGraphLaunch(gx, st_));
             download_states();
             std::vector<int> fold, rel;
             std::vector<std::pair<int, int>> drops;
@@ -426,8 +438,9 @@ private:
         download_states();  // keep the host copy current (rr, r2max set on the device)
     }
 
-    // One iteration on the runner's stream (what the graph holds).
-    void iteration()
+    // One iteration on the runner's stream (what the graphs hold), with the
+    // hopping term blocked nr right-hand sides per thread.
+    void iteration(int nr)
     {
         const CgmLaunch L = launch();
         CgmState<S> *st = state_.as<CgmState<S>>();
@@ -443,28 +456,32 @@ private:
         d.dst_parity = 1 - parity_;
         d.src = vec_p0();
         d.dst = vec(tmp_);
-        launch_dslash(d, EPI_STORE, tune_.nr, tune_.nt, st_);
+        launch_dslash(d, EPI_STORE, nr, tune_.nt, st_);
         d.dst_parity = parity_;
         d.src = vec(tmp_);
         d.dst = vec(s_);
         d.aux = vec_p0();
         d.sigma = sigma_.as<double>();
         d.partial = part_a_.as<double>();
-        launch_dslash(d, EPI_AXMY_DOT, tune_.nr, tune_.nt, st_);
+        launch_dslash(d, EPI_AXMY_DOT, nr, tune_.nt, st_);
         cgm_scalars1<S>(st, L, st_);
         cgm_axpy_norm<S, VR>(st, vec(r_), vec(s_), L, st_);
         cgm_scalars2<S>(st, L, st_);
         cgm_xp_update<S, VR, VR>(st, x_.as<VR>(), p_.as<VR>(), vec(r_), L, st_);
     }
 
-    void capture()
+    hipGraphExec_t graph_for(int nr)
     {
-        hipGraph_t graph;
-        KQ_HIP_CHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
-        iteration();
-        KQ_HIP_CHECK(hipStreamEndCapture(st_, &graph));
-        KQ_HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0));
-        KQ_HIP_CHECK(hipGraphDestroy(graph));
+        const int i = nr >= 8 ? 3 : nr >= 4 ? 2 : nr >= 2 ? 1 : 0;
+        if (!graphs_[i]) {
+            hipGraph_t graph;
+            KQ_HIP_CHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+            iteration(1 << i);
+            KQ_HIP_CHECK(hipStreamEndCapture(st_, &graph));
+            KQ_HIP_CHECK(hipGraphInstantiate(&graphs_[i], graph, nullptr, nullptr, 0));
+            KQ_HIP_CHECK(hipGraphDestroy(graph));
+        }
+        return graphs_[i];
     }
 
     Geom g_;
@@ -491,7 +508,7 @@ private:
     double delta_ = 0.1, pause2_ = 0.0;
     DevBuf b_hi_, x_hi_, tmp64_, ax64_;
     int reliable_updates_ = 0;
-    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 }  // namespace kq
