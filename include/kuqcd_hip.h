@@ -58,6 +58,8 @@ typedef struct {
     double fork_tol;       /* mixed modes: a shift k >= 1 leaves the multishift and continues as its own
                               FP64-reliable CG once its iterated relative residual is below fork_tol
                               (0: automatic, 1e-5 for FP32 and 1e-3 for FP16 links; < 0: never) */
+    double learned_tol;    /* learned = 1: relative residual at which shift 0 leaves the multishift for
+                              the flexible-CG refinement (0: 1e-6) */
 } KqSolveOptions;
 
 #define KQ_MAX_SHIFT 32
@@ -128,6 +130,101 @@ KQ_API void *kq_device_alloc(size_t bytes);
 KQ_API void kq_device_free(void *p);
 /* copy `bytes` between any two host/device pointers; returns 0 on success */
 KQ_API int kq_memcpy(void *dst, const void *src, size_t bytes);
+
+/* ---- learned preconditioner (no reference counterpart)
+ * A two-layer parallel-transport convolution M built from the loaded fat and
+ * long links: 16 taps (4 directions x forward/backward x hops 1 and 3), 4
+ * hidden colour-vector channels, complex weights applied in bfloat16 with
+ * FP32 accumulation:
+ *   h_c = sum_t W1[c][t] T_t r  (other parity),  z = W0 r + sum_t T_t sum_c W2[c][t] h_c
+ * tap t = mu*4 + 2*(backward) + (hop 3). Weights are in units of the links'
+ * root-mean-square row norm nu (per hop), i.e. the kernels use bf16(W / nu_hop).
+ * Trained to minimise the energy-norm error left after one step along z = M r,
+ *   L(r, e) = 1 - |<z, r>|^2 / (<z, A z> <r, e>),  e = A^{-1} r,
+ * on (residual, error) pairs that the solver generates. */
+typedef struct {
+    float w1[4][16][2]; /* [channel][tap][re, im] */
+    float w2[4][16][2];
+    float w0[2];
+} KqPtWeights;
+
+/* W0 = 1, W1 and W2 complex Gaussian with standard deviation `scale` (host only) */
+KQ_API void kq_ptnet_init_weights(KqPtWeights *w, uint64_t seed, double scale);
+/* weights used by the learned refinement (kq_solve_multishift_ex with learned = 1, kq_fcg_refine) */
+KQ_API KuQCDStatus kq_ptnet_set_weights(const KqPtWeights *w);
+KQ_API KuQCDStatus kq_ptnet_get_weights(KqPtWeights *w);
+/* link norms nu[0] (fat), nu[1] (long) of the loaded double gauge pair */
+KQ_API KuQCDStatus kq_ptnet_norms(double nu[2]);
+/* z = M r on parity `parity` with the loaded gauge (r, z: reference layout, double, host or device);
+ * h (may be NULL): the 4 hidden channels, each a reference-layout vector on parity 1-parity.
+ * link_format: KQ_MODE_MIXED_FP32 or KQ_MODE_MIXED_FP16 */
+KQ_API KuQCDStatus kq_ptnet_forward(const KqPtWeights *w, int parity, int link_format, const void *r, void *z,
+                                    void *h);
+/* loss L(r, e) for A = sigma - D D and its gradient dL/dconj(W) with respect to the weights as
+ * stored in KqPtWeights (258 doubles: w1, w2, w0 as re, im pairs); FP32 links and vectors */
+KQ_API KuQCDStatus kq_ptnet_loss_grad(const KqPtWeights *w, int parity, double sigma, const void *r, const void *e,
+                                      double *loss, double *grad);
+
+/* Training set: each call adds the loaded gauge pair (FP32 copy) and, for `nsrc` Gaussian sources
+ * (seeds seed, seed+1, ...), the pairs (r_k, x* - x_k) of an FP64-reliable CG on A = sigma - D D
+ * from x_0 = 0 down to relative residual rel_tol, taken at every reliable update whose relative
+ * residual is at least min_level; vectors are normalised to |r_k| = 1. *nsamples: total so far. */
+KQ_API KuQCDStatus kq_learned_collect(int parity, double sigma, int nsrc, uint64_t seed, double rel_tol,
+                                      double min_level, int *nsamples);
+KQ_API void kq_learned_clear(void);
+typedef struct {
+    int steps;        /* optimizer steps */
+    int batch;        /* samples per step */
+    double lr;        /* Adam step size at the start (cosine decay to lr_final) */
+    double lr_final;
+    uint64_t seed;    /* sample order */
+    int init;         /* 1: start from kq_ptnet_init_weights(seed, init_scale); 0: from the current weights */
+    double init_scale;
+    int verbose;
+} KqTrainOptions;
+#define KQ_TRAIN_HIST 64
+typedef struct {
+    int steps, nsamples, ngauges;
+    double loss_initial;            /* mean loss over the training set before training */
+    double loss_final;              /* ... and after */
+    double loss_hist[KQ_TRAIN_HIST]; /* mean batch loss, sampled evenly over the steps */
+    double seconds;
+} KqTrainReport;
+/* Adam on the training set; the result becomes the current weights */
+KQ_API KuQCDStatus kq_learned_train(const KqTrainOptions *o, KqTrainReport *rep);
+/* mean loss of weights w over the training set */
+KQ_API KuQCDStatus kq_learned_eval(const KqPtWeights *w, double *mean_loss);
+
+/* Flexible CG for one shifted system A = sigma - D D on parity `parity` with the loaded gauge:
+ * x (reference layout, double, host or device) is refined in place until ||b - A x|| <= rel_tol ||b||
+ * (FP64 true residual). FP32 or FP16 links and FP32 vectors inside, FP64 reliable updates every
+ * factor `delta` of residual reduction. With learned = 1 the current weights precondition it; if a
+ * reliable update finds the FP64 residual not reduced, or no reduction by delta within stall_iter
+ * iterations, it falls back to plain CG from the best iterate. */
+typedef struct {
+    double rel_tol;
+    int learned;
+    int mode;         /* KQ_MODE_MIXED_FP32 or KQ_MODE_MIXED_FP16: link format of A */
+    int net_format;   /* link format of the network (0: FP16) */
+    int max_iter;
+    double delta;     /* 0: 0.1 */
+    int chunk;        /* iterations per graph replay (0: 8) */
+    int stall_iter;   /* 0: 2000 */
+    int verbose;
+} KqRefineOptions;
+typedef struct {
+    int iter;             /* flexible-CG iterations in total */
+    int iter_learned;     /* of which with the network */
+    int reliable_updates;
+    int fallback;         /* 1 if the network was dropped for plain CG */
+    int converged;
+    double rel_res_initial;
+    double true_rel_res;  /* FP64 ||b - A x|| / ||b|| at the end */
+    double seconds;       /* device-resident part (without host copies) */
+} KqRefineReport;
+KQ_API void kq_default_refine_options(KqRefineOptions *o);
+KQ_API KuQCDStatus kq_fcg_refine(int parity, double sigma, const void *b, void *x, const KqRefineOptions *o,
+                                 KqRefineReport *rep);
 
 /* ---- misc */
 KQ_API const char *kq_last_error(void);

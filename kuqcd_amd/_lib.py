@@ -77,6 +77,7 @@ class KqSolveOptions(C.Structure):
         ("nt", C.c_int),
         ("verbose", C.c_int),
         ("fork_tol", C.c_double),
+        ("learned_tol", C.c_double),
     ]
 
 
@@ -109,6 +110,70 @@ class KqBenchResult(C.Structure):
     ]
 
 
+class KqPtWeights(C.Structure):
+    _fields_ = [
+        ("w1", ((C.c_float * 2) * 16) * 4),
+        ("w2", ((C.c_float * 2) * 16) * 4),
+        ("w0", C.c_float * 2),
+    ]
+
+
+KQ_TRAIN_HIST = 64
+KQ_PT_PARAMS = 2 * 4 * 16 + 1  # complex weights
+
+
+class KqTrainOptions(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int),
+        ("batch", C.c_int),
+        ("lr", C.c_double),
+        ("lr_final", C.c_double),
+        ("seed", C.c_uint64),
+        ("init", C.c_int),
+        ("init_scale", C.c_double),
+        ("verbose", C.c_int),
+    ]
+
+
+class KqTrainReport(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int),
+        ("nsamples", C.c_int),
+        ("ngauges", C.c_int),
+        ("loss_initial", C.c_double),
+        ("loss_final", C.c_double),
+        ("loss_hist", C.c_double * KQ_TRAIN_HIST),
+        ("seconds", C.c_double),
+    ]
+
+
+class KqRefineOptions(C.Structure):
+    _fields_ = [
+        ("rel_tol", C.c_double),
+        ("learned", C.c_int),
+        ("mode", C.c_int),
+        ("net_format", C.c_int),
+        ("max_iter", C.c_int),
+        ("delta", C.c_double),
+        ("chunk", C.c_int),
+        ("stall_iter", C.c_int),
+        ("verbose", C.c_int),
+    ]
+
+
+class KqRefineReport(C.Structure):
+    _fields_ = [
+        ("iter", C.c_int),
+        ("iter_learned", C.c_int),
+        ("reliable_updates", C.c_int),
+        ("fallback", C.c_int),
+        ("converged", C.c_int),
+        ("rel_res_initial", C.c_double),
+        ("true_rel_res", C.c_double),
+        ("seconds", C.c_double),
+    ]
+
+
 _VP = C.c_void_p
 _PROTOS = {
     "kq_init": (C.c_int, []),
@@ -134,6 +199,21 @@ _PROTOS = {
     "kq_device_alloc": (_VP, [C.c_size_t]),
     "kq_device_free": (None, [_VP]),
     "kq_memcpy": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "kq_ptnet_init_weights": (None, [C.POINTER(KqPtWeights), C.c_uint64, C.c_double]),
+    "kq_ptnet_set_weights": (C.c_int, [C.POINTER(KqPtWeights)]),
+    "kq_ptnet_get_weights": (C.c_int, [C.POINTER(KqPtWeights)]),
+    "kq_ptnet_norms": (C.c_int, [C.POINTER(C.c_double)]),
+    "kq_ptnet_forward": (C.c_int, [C.POINTER(KqPtWeights), C.c_int, C.c_int, _VP, _VP, _VP]),
+    "kq_ptnet_loss_grad": (C.c_int, [C.POINTER(KqPtWeights), C.c_int, C.c_double, _VP, _VP,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "kq_learned_collect": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_uint64, C.c_double, C.c_double,
+                                     C.POINTER(C.c_int)]),
+    "kq_learned_clear": (None, []),
+    "kq_learned_train": (C.c_int, [C.POINTER(KqTrainOptions), C.POINTER(KqTrainReport)]),
+    "kq_learned_eval": (C.c_int, [C.POINTER(KqPtWeights), C.POINTER(C.c_double)]),
+    "kq_default_refine_options": (None, [C.POINTER(KqRefineOptions)]),
+    "kq_fcg_refine": (C.c_int, [C.c_int, C.c_double, _VP, _VP, C.POINTER(KqRefineOptions),
+                                C.POINTER(KqRefineReport)]),
     "kq_last_error": (C.c_char_p, []),
     "kq_set_verbosity": (None, [C.c_int]),
     "kq_device_count": (C.c_int, []),

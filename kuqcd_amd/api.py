@@ -207,3 +207,117 @@ def set_verbosity(v: int) -> None:
 
 def device_count() -> int:
     return L.lib().kq_device_count()
+
+
+# ---------------------------------------------------------------- learned preconditioner
+def weights_to_array(w: L.KqPtWeights) -> tuple[np.ndarray, np.ndarray, complex]:
+    """(W1 [4,16] complex, W2 [4,16] complex, W0) of a KqPtWeights struct."""
+    a = np.frombuffer(bytes(w), dtype=np.float32).astype(np.float64)
+    c = a[0::2] + 1j * a[1::2]
+    return c[:64].reshape(4, 16), c[64:128].reshape(4, 16), complex(c[128])
+
+
+def weights_from_array(w1, w2, w0) -> L.KqPtWeights:
+    c = np.concatenate([np.asarray(w1, np.complex128).ravel(), np.asarray(w2, np.complex128).ravel(),
+                        np.asarray([w0], np.complex128)])
+    f = np.empty(2 * c.size, dtype=np.float32)
+    f[0::2], f[1::2] = c.real, c.imag
+    w = L.KqPtWeights()
+    C.memmove(C.addressof(w), f.ctypes.data, C.sizeof(w))
+    return w
+
+
+def ptnet_init_weights(seed: int = 1, scale: float = 0.05) -> L.KqPtWeights:
+    w = L.KqPtWeights()
+    L.lib().kq_ptnet_init_weights(C.byref(w), seed, scale)
+    return w
+
+
+def ptnet_set_weights(w: L.KqPtWeights) -> int:
+    return L.lib().kq_ptnet_set_weights(C.byref(w))
+
+
+def ptnet_get_weights() -> L.KqPtWeights:
+    w = L.KqPtWeights()
+    st = L.lib().kq_ptnet_get_weights(C.byref(w))
+    if st != L.KUQCD_SUCCESS:
+        raise RuntimeError(f"kq_ptnet_get_weights: {st} {L.last_error()}")
+    return w
+
+
+def ptnet_norms() -> tuple[float, float]:
+    nu = (C.c_double * 2)()
+    st = L.lib().kq_ptnet_norms(nu)
+    if st != L.KUQCD_SUCCESS:
+        raise RuntimeError(f"kq_ptnet_norms: {st} {L.last_error()}")
+    return nu[0], nu[1]
+
+
+def ptnet_forward(w: L.KqPtWeights, r: np.ndarray, parity: int = 0, link_format: int = L.KQ_MODE_MIXED_FP32,
+                  hidden: bool = False):
+    """z = M r (and the hidden channels, shape (4, Vh, 3, 2), if hidden)."""
+    z = np.empty_like(r)
+    h = np.empty((4,) + r.shape, dtype=np.float64) if hidden else None
+    st = L.lib().kq_ptnet_forward(C.byref(w), parity, link_format, _ptr(r), _ptr(z), _ptr(h))
+    if st != L.KUQCD_SUCCESS:
+        raise RuntimeError(f"kq_ptnet_forward: {st} {L.last_error()}")
+    return (z, h) if hidden else z
+
+
+def ptnet_loss_grad(w: L.KqPtWeights, r: np.ndarray, e: np.ndarray, sigma: float, parity: int = 0):
+    """(loss, dL/dconj(W) as complex [129]: W1 (64), W2 (64), W0)."""
+    loss = C.c_double()
+    g = np.empty(2 * L.KQ_PT_PARAMS, dtype=np.float64)
+    st = L.lib().kq_ptnet_loss_grad(C.byref(w), parity, sigma, _ptr(r), _ptr(e), C.byref(loss),
+                                    g.ctypes.data_as(C.POINTER(C.c_double)))
+    if st != L.KUQCD_SUCCESS:
+        raise RuntimeError(f"kq_ptnet_loss_grad: {st} {L.last_error()}")
+    return loss.value, g[0::2] + 1j * g[1::2]
+
+
+def learned_collect(parity: int, sigma: float, nsrc: int = 1, seed: int = 1, rel_tol: float = 1e-11,
+                    min_level: float = 1e-8) -> int:
+    n = C.c_int()
+    st = L.lib().kq_learned_collect(parity, sigma, nsrc, seed, rel_tol, min_level, C.byref(n))
+    if st != L.KUQCD_SUCCESS:
+        raise RuntimeError(f"kq_learned_collect: {st} {L.last_error()}")
+    return n.value
+
+
+def learned_clear() -> None:
+    L.lib().kq_learned_clear()
+
+
+def learned_train(steps=300, batch=4, lr=0.02, lr_final=0.002, seed=1, init=1, init_scale=0.05,
+                  verbose=0) -> L.KqTrainReport:
+    o = L.KqTrainOptions(steps=steps, batch=batch, lr=lr, lr_final=lr_final, seed=seed, init=init,
+                         init_scale=init_scale, verbose=verbose)
+    rep = L.KqTrainReport()
+    st = L.lib().kq_learned_train(C.byref(o), C.byref(rep))
+    if st != L.KUQCD_SUCCESS:
+        raise RuntimeError(f"kq_learned_train: {st} {L.last_error()}")
+    return rep
+
+
+def learned_eval(w: L.KqPtWeights) -> float:
+    m = C.c_double()
+    st = L.lib().kq_learned_eval(C.byref(w), C.byref(m))
+    if st != L.KUQCD_SUCCESS:
+        raise RuntimeError(f"kq_learned_eval: {st} {L.last_error()}")
+    return m.value
+
+
+def refine_options(**kw) -> L.KqRefineOptions:
+    o = L.KqRefineOptions()
+    L.lib().kq_default_refine_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def fcg_refine(b: np.ndarray, x: np.ndarray, sigma: float, parity: int = 0, opt: L.KqRefineOptions | None = None):
+    """Refine x in place for (sigma - D D) x = b; returns (status, KqRefineReport)."""
+    opt = opt if opt is not None else refine_options()
+    rep = L.KqRefineReport()
+    st = L.lib().kq_fcg_refine(parity, sigma, _ptr(b), _ptr(x), C.byref(opt), C.byref(rep))
+    return st, rep

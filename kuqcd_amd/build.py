@@ -27,6 +27,7 @@ SOURCES = [
     "kq_cgm_kernels.hip",
     "kq_layout.hip",
     "kq_ptnet.hip",
+    "kq_fcg.hip",
     "kq_api.cpp",
     "kq_solve.cpp",
     "kq_learned.cpp",
@@ -66,7 +67,10 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     """Compile every source and link libkuqcd.so; returns its path."""
     OBJDIR.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
-    srcs = [s for s in SOURCES if (CSRC / s).exists()]
+    missing = [s for s in SOURCES if not (CSRC / s).exists()]
+    if missing:
+        raise FileNotFoundError(f"missing sources: {missing}")
+    srcs = list(SOURCES)
     jobs = jobs or min(len(srcs), max(1, min(16, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "kq_context.h"
+#include "kq_learned.h"
 #include "kq_solve.h"
 #include "kuqcd.h"
 #include "kuqcd_hip.h"
@@ -453,6 +454,7 @@ void kq_end(void)
         Context &c = ctx();
         if (c.device_ready)
             (void)hipStreamSynchronize(c.stream);
+        learned_release();
     } catch (...) {
     }
 }
@@ -471,6 +473,7 @@ void kq_default_options(KqSolveOptions *opt)
     opt->nt = 0;
     opt->verbose = 1;
     opt->fork_tol = 0.0;
+    opt->learned_tol = 0.0;
 }
 
 KuQCDStatus kq_solve_multishift_ex(void **x, void *b, KuQCDInvertParam *param, const KqSolveOptions *opt,

@@ -2,6 +2,7 @@
 // FP64 / mixed-precision multishift CG with per-shift FP64 verification and
 // batched refinement.
 #include "kq_solve.h"
+#include "kq_learned.h"
 
 #include <algorithm>
 #include <chrono>
@@ -26,27 +27,6 @@ std::vector<double> reduce_host(const double *part_dev, uint32_t nblk, int n, hi
     KQ_HIP_CHECK(hipMemcpyAsync(h.data(), out.as<void>(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
     KQ_HIP_CHECK(hipStreamSynchronize(st));
     return h;
-}
-
-// caller vector (reference layout, HR; host or device memory) -> device SoA vector r of v
-template <typename HR, typename VR> static void upload_vec(const void *h, Vec<VR> v, int r, DevBuf &stage, hipStream_t st)
-{
-    const size_t n = (size_t)6 * v.Vh;
-    if (stage.bytes() < sizeof(HR) * n)
-        stage.alloc(sizeof(HR) * n);
-    KQ_HIP_CHECK(hipMemcpyAsync(stage.as<void>(), h, sizeof(HR) * n, hipMemcpyDefault, st));
-    vec_host_to_soa<HR, VR>(stage.as<HR>(), v, r, st);
-    KQ_HIP_CHECK(hipStreamSynchronize(st));
-}
-
-template <typename VR, typename HR> static void download_vec(Vec<VR> v, int r, void *h, DevBuf &stage, hipStream_t st)
-{
-    const size_t n = (size_t)6 * v.Vh;
-    if (stage.bytes() < sizeof(HR) * n)
-        stage.alloc(sizeof(HR) * n);
-    vec_soa_to_host<VR, HR>(v, r, stage.as<HR>(), st);
-    KQ_HIP_CHECK(hipMemcpyAsync(h, stage.as<void>(), sizeof(HR) * n, hipMemcpyDefault, st));
-    KQ_HIP_CHECK(hipStreamSynchronize(st));
 }
 
 static void print_iters(int iter, int max_iter)
@@ -257,11 +237,16 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
     std::vector<int> sys_of(nrhs);
     std::vector<int> rhs_of(max_sys, -1);
     std::vector<int> forked((size_t)nrhs * ns, -1);
+    const bool learned = P.o->learned != 0;
+    const double ltol = P.o->learned_tol > 0 ? P.o->learned_tol : 1e-6;
     for (int r = 0; r < nrhs; ++r) {
         std::vector<double> tolk(ns, P.tol[r]);
         if (fork > 0)
             for (int k = 1; k < ns; ++k)
                 tolk[k] = std::max(P.tol[r], fork * fork * P.b2[r]);
+        // shift 0 leaves for the flexible-CG refinement at learned_tol
+        if (learned)
+            tolk[0] = std::max(P.tol[r], ltol * ltol * P.b2[r]);
         sys_of[r] = run.add_system(ns, P.p->shifts + (size_t)r * ns, tolk.data(), P.p->max_iter,
                                    P.b->as<double>() + (size_t)r * 2 * n3, nullptr);
         rhs_of[sys_of[r]] = r;
@@ -291,6 +276,28 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
     auto t1 = clk::now();
     for (int r = 0; r < nrhs; ++r)
         rep[r].seconds_multishift = secs(t0, t1);
+
+    // learned-preconditioner flexible CG for shift 0, from the multishift iterate
+    double t_learned = 0.0;
+    if (learned) {
+        KqRefineOptions ro;
+        kq_default_refine_options(&ro);
+        ro.mode = P.o->mode;
+        ro.learned = 1;
+        ro.max_iter = P.p->max_iter > 0 ? P.p->max_iter : 1;
+        ro.delta = P.delta;
+        ro.verbose = P.o->verbose;
+        for (int r = 0; r < nrhs; ++r) {
+            ro.rel_tol = 0.9 * P.eps[r];
+            const RefineResult rr = fcg_refine_device(P.slot, P.parity, abs_shifts(*P.p, r)[0],
+                                                      P.b->as<double>() + (size_t)r * 2 * n3,
+                                                      x_out.as<double>() + (size_t)r * ns * 2 * n3, ro);
+            rep[r].refine_iter[0] = rr.iter;
+            rep[r].learned_used = rr.fallback ? 0 : 1;
+            rep[r].learned_fallback = rr.fallback ? 1 : 0;
+            t_learned += rr.seconds;
+        }
+    }
 
     // per-shift FP64 verification; refine what misses the target
     res2.assign((size_t)nrhs * ns, 0.0);
@@ -338,7 +345,7 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
     }
     auto t3 = clk::now();
     for (int r = 0; r < nrhs; ++r) {
-        rep[r].seconds_refine = secs(t2, t3) - (t_verify - 0.0);
+        rep[r].seconds_refine = secs(t2, t3) - (t_verify - 0.0) + t_learned;
         rep[r].seconds_verify = t_verify;
     }
 }
@@ -416,8 +423,8 @@ KuQCDStatus solve_extended(int slot, int nrhs, void **x, void **b, KuQCDInvertPa
         set_error("the FP64 and mixed-precision modes need a double-precision gauge field");
         return KUQCD_ERR_UNSUPPORTED;
     }
-    if (o.learned) {
-        set_error("the learned refinement is not available in this build");
+    if (o.learned && o.mode != KQ_MODE_FP64 && o.mode != KQ_MODE_REFERENCE && !learned_weights_available()) {
+        set_error("no preconditioner weights: call kq_ptnet_set_weights or kq_learned_train first");
         return KUQCD_ERR_UNSUPPORTED;
     }
     Problem P;

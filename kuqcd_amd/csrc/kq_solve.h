@@ -17,6 +17,27 @@ KuQCDStatus solve_extended(int slot, int nrhs, void **x, void **b, KuQCDInvertPa
 // sum_blk part[k][blk] on the host side
 std::vector<double> reduce_host(const double *part_dev, uint32_t nblk, int n, hipStream_t st);
 
+// caller vector (reference layout, HR; host or device memory) -> device SoA vector r of v
+template <typename HR, typename VR> inline void upload_vec(const void *h, Vec<VR> v, int r, DevBuf &stage, hipStream_t st)
+{
+    const size_t n = (size_t)6 * v.Vh;
+    if (stage.bytes() < sizeof(HR) * n)
+        stage.alloc(sizeof(HR) * n);
+    KQ_HIP_CHECK(hipMemcpyAsync(stage.as<void>(), h, sizeof(HR) * n, hipMemcpyDefault, st));
+    vec_host_to_soa<HR, VR>(stage.as<HR>(), v, r, st);
+    KQ_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+template <typename VR, typename HR> inline void download_vec(Vec<VR> v, int r, void *h, DevBuf &stage, hipStream_t st)
+{
+    const size_t n = (size_t)6 * v.Vh;
+    if (stage.bytes() < sizeof(HR) * n)
+        stage.alloc(sizeof(HR) * n);
+    vec_soa_to_host<VR, HR>(v, r, stage.as<HR>(), st);
+    KQ_HIP_CHECK(hipMemcpyAsync(h, stage.as<void>(), sizeof(HR) * n, hipMemcpyDefault, st));
+    KQ_HIP_CHECK(hipStreamSynchronize(st));
+}
+
 // autotuned (nr, nt) for the hopping term
 std::pair<int, int> tuned_launch(int slot, int mode, int nrhs);
 
