@@ -58,6 +58,8 @@ def main() -> int:
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only launch the hopping terms (FP64, FP32, FP16) and the copy kernel, for counter runs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -79,6 +81,16 @@ def main() -> int:
     b = A.gen_source(lat, 0, seed=3)
     log(f"inputs generated in {time.time() - t:.1f}s")
     A.load_gauge_pair(fat, lng, lat)
+    if args.kernels_only:
+        for mode, name in ((1, "fp64"), (2, "fp32"), (3, "fp16")):
+            st, r = A.bench_dslash(mode, 1, 0, 0, 20)
+            assert st == 0, A.last_error()
+            print(f"dslash {name}: {r.ms * 1e3:.1f} us, {r.bytes / Vh:.0f} B/site, {r.gbs:.0f} GB/s, "
+                  f"nr {r.nr} nt {r.nt}", flush=True)
+        st, cp = A.bench_copy(1 << 30, 10)
+        print(f"copy: 1 GiB read + 1 GiB written per launch, {cp.ms * 1e3:.1f} us, {cp.gbs:.0f} GB/s", flush=True)
+        A.kuqcd_gauge_free()
+        return 0
 
     s = np.logspace(-4, 0, args.nshift)
     m2 = args.mass ** 2
