@@ -100,7 +100,17 @@ DerivedFormats &derived(int slot, bool want_f32, bool want_f16)
         d.lng_f.alloc(link_field_bytes<LinkF>(Vh));
         canon_to_linkF<double>(f.soa.as<double2>(), d.fat_f.as<char>(), Vh, c.stream);
         canon_to_linkF<double>(l.soa.as<double2>(), d.lng_f.as<char>(), Vh, c.stream);
+        // long links also as 12 reals when every one is b * SU(3)
+        DevBuf fail(sizeof(int));
+        KQ_HIP_CHECK(hipMemsetAsync(fail.as<void>(), 0, sizeof(int), c.stream));
+        d.lng_f12.alloc(link_field_bytes<LinkF12>(Vh));
+        canon_to_linkF12(l.soa.as<double2>(), d.lng_f12.as<char>(), Vh, fail.as<int>(), c.stream);
+        int nfail = 0;
+        KQ_HIP_CHECK(hipMemcpyAsync(&nfail, fail.as<void>(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
         KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+        d.f32_long12 = (nfail == 0);
+        if (!d.f32_long12)
+            d.lng_f12.release();
         d.f32 = true;
     }
     if (want_f16 && !d.f16) {

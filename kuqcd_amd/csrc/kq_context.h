@@ -9,6 +9,7 @@
 
 #include "kq_solver.h"
 #include "kuqcd_enum.h"
+#include "kuqcd_hip.h"
 
 namespace kq {
 
@@ -27,14 +28,17 @@ struct GaugeSlot {
 // Lower-precision kernel formats of a double gauge pair, made on first use.
 struct DerivedFormats {
     bool f32 = false, f16 = false;
-    bool long12 = false;  // long links stored as 12 reals (LinkH12), else LinkH
-    DevBuf fat_f, lng_f;  // LinkF
-    DevBuf fat_h, lng_h;  // LinkH / LinkH12
+    bool long12 = false;      // FP16 long links stored as 12 reals (LinkH12), else LinkH
+    bool f32_long12 = false;  // lng_f12 holds the long links as LinkF12
+    DevBuf fat_f, lng_f;      // LinkF
+    DevBuf lng_f12;           // LinkF12 (when f32_long12)
+    DevBuf fat_h, lng_h;      // LinkH / LinkH12
 };
 
 inline LinkF as_linkF(const DevBuf &b, uint32_t Vh) { return LinkF{b.as<char>(), nb64_of(Vh)}; }
 inline LinkH as_linkH(const DevBuf &b, uint32_t Vh) { return LinkH{b.as<char>(), nb64_of(Vh)}; }
 inline LinkH12 as_linkH12(const DevBuf &b, uint32_t Vh) { return LinkH12{b.as<char>(), nb64_of(Vh)}; }
+inline LinkF12 as_linkF12(const DevBuf &b, uint32_t Vh) { return LinkF12{b.as<char>(), nb64_of(Vh)}; }
 
 struct Context {
     bool device_ready = false;
@@ -61,6 +65,31 @@ LinkD link_d(const GaugeSlot &s);
 LinkF link_f(const GaugeSlot &s);
 // derived formats of a double slot (created on demand)
 DerivedFormats &derived(int slot, bool want_f32, bool want_f16);
+
+// fn(fat, lng) with the device link formats of a mixed mode of a double slot:
+// KQ_MODE_MIXED_FP32 -> LinkF fat, LinkF12 long links if they are b * SU(3) else LinkF;
+// KQ_MODE_MIXED_FP16 -> LinkH fat, LinkH12 long links if they are b * SU(3) else LinkH.
+// Returns whether the long links use 12 reals.
+template <class Fn> bool with_mixed_links(int slot, int mode, Fn &&fn)
+{
+    const uint32_t Vh = slot_geom(slot).Vh;
+    if (mode == KQ_MODE_MIXED_FP16) {
+        DerivedFormats &d = derived(slot, false, true);
+        if (d.long12)
+            fn(as_linkH(d.fat_h, Vh), as_linkH12(d.lng_h, Vh));
+        else
+            fn(as_linkH(d.fat_h, Vh), as_linkH(d.lng_h, Vh));
+        return d.long12;
+    }
+    if (mode != KQ_MODE_MIXED_FP32)
+        throw std::invalid_argument("link format must be KQ_MODE_MIXED_FP32 or KQ_MODE_MIXED_FP16");
+    DerivedFormats &d = derived(slot, true, false);
+    if (d.f32_long12)
+        fn(as_linkF(d.fat_f, Vh), as_linkF12(d.lng_f12, Vh));
+    else
+        fn(as_linkF(d.fat_f, Vh), as_linkF(d.lng_f, Vh));
+    return d.f32_long12;
+}
 
 void log_info(const char *fmt, ...);
 void set_error(const std::string &msg);

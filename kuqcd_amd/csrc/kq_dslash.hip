@@ -200,6 +200,7 @@ void launch_dslash(DslashArgs<LF, LL, SR> a, int epi, int nr, int nt, hipStream_
 
 template void launch_dslash<LinkD, LinkD, double>(DslashArgs<LinkD, LinkD, double>, int, int, int, hipStream_t);
 template void launch_dslash<LinkF, LinkF, float>(DslashArgs<LinkF, LinkF, float>, int, int, int, hipStream_t);
+template void launch_dslash<LinkF, LinkF12, float>(DslashArgs<LinkF, LinkF12, float>, int, int, int, hipStream_t);
 template void launch_dslash<LinkH, LinkH, float>(DslashArgs<LinkH, LinkH, float>, int, int, int, hipStream_t);
 template void launch_dslash<LinkH, LinkH12, float>(DslashArgs<LinkH, LinkH12, float>, int, int, int, hipStream_t);
 

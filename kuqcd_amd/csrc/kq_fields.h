@@ -152,6 +152,41 @@ struct LinkH12 {
     }
 };
 
+// FP32, 12 reals (rows 0 and 1 of an SU(3) matrix V), one signed FP32 scale b
+// per link: U = b * V with row 2 of V = conj(row0 x row1).
+// Chunks: 3 x 16 B (complex 0..5), 4 B (scale).
+struct LinkF12 {
+    using R = float;
+    static constexpr int kBytes = 52;
+    static constexpr size_t kBlock = 64 * kBytes;
+    const char *p;
+    uint32_t nb64;
+    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
+    {
+        const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
+        const uint32_t l = s & 63;
+        float2 v[6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const kq_v4f w = ld_nt(reinterpret_cast<const kq_v4f *>(b + 1024 * c) + l);
+            v[2 * c] = mk<float2>(w.x, w.y);
+            v[2 * c + 1] = mk<float2>(w.z, w.w);
+        }
+        const float a = ld_nt(reinterpret_cast<const float *>(b + 3072) + l);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int c1i = (c + 1) % 3, c2i = (c + 2) % 3;
+            const float2 a1 = v[c1i], b2 = v[3 + c2i], a2 = v[c2i], b1 = v[3 + c1i];
+            const float re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
+            const float im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
+            u[6 + c] = mk<float2>(re * a, -im * a);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            u[k] = mk<float2>(v[k].x * a, v[k].y * a);
+    }
+};
+
 inline uint32_t nb64_of(uint32_t Vh) { return (Vh + 63) / 64; }
 // bytes of one link field (both parities, 4 directions) in format LK
 template <class LK> inline size_t link_field_bytes(uint32_t Vh) { return (size_t)8 * nb64_of(Vh) * LK::kBlock; }

@@ -27,6 +27,8 @@ void canon_to_linkH(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st
 // FP64 links of the form b * V (V in SU(3), b real) -> LinkH12: rows 0,1 of V in
 // FP16 and b in FP32. fail[0] counts links that are not of that form to 1e-6.
 void canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);
+// LinkF12 from canonical double links; *fail counts links that are not b * SU(3)
+void canon_to_linkF12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);
 
 // host-layout colour vector (device copy) <-> SoA vector of right-hand side r
 template <typename HR, typename VR>

@@ -141,20 +141,17 @@ __device__ inline double2 det3(const double2 m[9])
     return add(add(mul(m[0], c0), mul(m[1], c1)), mul(m[2], c2));
 }
 
-__global__ void k_canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64, int *fail)
+// V = U / b with b = cbrt(Re det U), checked to be unitary; false if U is not b * SU(3)
+__device__ inline bool su3_factor(const double2 *canon, uint32_t qm, uint32_t s, uint32_t Vh, double2 v[9],
+                                  double &bsc)
 {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)8 * Vh)
-        return;
-    const uint32_t qm = (uint32_t)(t / Vh), s = (uint32_t)(t - (size_t)qm * Vh);
     double2 m[9];
     for (int k = 0; k < 9; ++k)
         m[k] = canon[((size_t)qm * 9 + k) * Vh + s];
     const double2 d = det3(m);
     // b^3 = det(b V) must be real; b = cbrt(Re det)
-    const double bsc = cbrt(d.x);
+    bsc = cbrt(d.x);
     bool ok = fabs(d.y) <= 1e-6 * fabs(d.x) && bsc != 0.0;
-    double2 v[9];
     for (int k = 0; k < 9; ++k)
         v[k] = ok ? mk<double2>(m[k].x / bsc, m[k].y / bsc) : mk<double2>(0.0, 0.0);
     if (ok) {
@@ -172,7 +169,18 @@ __global__ void k_canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh,
                     ok = false;
             }
     }
-    if (!ok)
+    return ok;
+}
+
+__global__ void k_canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64, int *fail)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)8 * Vh)
+        return;
+    const uint32_t qm = (uint32_t)(t / Vh), s = (uint32_t)(t - (size_t)qm * Vh);
+    double2 v[9];
+    double bsc;
+    if (!su3_factor(canon, qm, s, Vh, v, bsc))
         atomicAdd(fail, 1);
     unsigned int h[6];
     for (int k = 0; k < 6; ++k)
@@ -190,6 +198,34 @@ void canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh, int *fail, h
     if (n == 0)
         return;
     hipLaunchKernelGGL(k_canon_to_linkH12, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
+                       nb64_of(Vh), fail);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_canon_to_linkF12(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64, int *fail)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)8 * Vh)
+        return;
+    const uint32_t qm = (uint32_t)(t / Vh), s = (uint32_t)(t - (size_t)qm * Vh);
+    double2 v[9];
+    double bsc;
+    if (!su3_factor(canon, qm, s, Vh, v, bsc))
+        atomicAdd(fail, 1);
+    char *b = dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkF12::kBlock);
+    const uint32_t l = s & 63;
+    for (int c = 0; c < 3; ++c)
+        reinterpret_cast<float4 *>(b + 1024 * c)[l] =
+            make_float4((float)v[2 * c].x, (float)v[2 * c].y, (float)v[2 * c + 1].x, (float)v[2 * c + 1].y);
+    reinterpret_cast<float *>(b + 3072)[l] = (float)bsc;
+}
+
+void canon_to_linkF12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+{
+    const size_t n = (size_t)8 * Vh;
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_canon_to_linkF12, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
                        nb64_of(Vh), fail);
     KQ_HIP_CHECK(hipGetLastError());
 }

@@ -107,24 +107,7 @@ PtNorms slot_norms(int slot)
     return nu;
 }
 
-// fn(fat, lng) with the device link format of `mode` (KQ_MODE_MIXED_FP32 or _FP16)
-template <class Fn> void with_format(int slot, int mode, Fn &&fn)
-{
-    const Geom g = slot_geom(slot);
-    if (mode == KQ_MODE_MIXED_FP16) {
-        DerivedFormats &d = derived(slot, false, true);
-        const LinkH fh = as_linkH(d.fat_h, g.Vh);
-        if (d.long12)
-            fn(fh, as_linkH12(d.lng_h, g.Vh));
-        else
-            fn(fh, as_linkH(d.lng_h, g.Vh));
-    } else if (mode == KQ_MODE_MIXED_FP32) {
-        DerivedFormats &d = derived(slot, true, false);
-        fn(as_linkF(d.fat_f, g.Vh), as_linkF(d.lng_f, g.Vh));
-    } else {
-        throw std::invalid_argument("link format must be KQ_MODE_MIXED_FP32 or KQ_MODE_MIXED_FP16");
-    }
-}
+template <class Fn> void with_format(int slot, int mode, Fn &&fn) { (void)with_mixed_links(slot, mode, fn); }
 
 // ---------------------------------------------------------------- A = sigma - D D
 template <class LF, class LL, typename R> struct OpA {

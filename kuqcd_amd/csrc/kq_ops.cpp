@@ -108,21 +108,11 @@ static KuQCDStatus apply(int nrhs, const void *src, void *dst, int parity, int m
         set_error("mixed modes need a double-precision gauge field");
         return KUQCD_ERR_UNSUPPORTED;
     }
-    if (mode == KQ_MODE_MIXED_FP32) {
-        DerivedFormats &d = derived(slot, true, false);
-        apply_t<LinkF, LinkF, float, double>(as_linkF(d.fat_f, g.Vh), as_linkF(d.lng_f, g.Vh), g,
-                                             nrhs, src, dst, parity, nr, nt, mdaggm, sigma);
-        return KUQCD_SUCCESS;
-    }
-    if (mode == KQ_MODE_MIXED_FP16) {
-        DerivedFormats &d = derived(slot, false, true);
-        LinkH fh = as_linkH(d.fat_h, g.Vh);
-        if (d.long12)
-            apply_t<LinkH, LinkH12, float, double>(fh, as_linkH12(d.lng_h, g.Vh), g,
-                                                   nrhs, src, dst, parity, nr, nt, mdaggm, sigma);
-        else
-            apply_t<LinkH, LinkH, float, double>(fh, as_linkH(d.lng_h, g.Vh), g, nrhs,
-                                                 src, dst, parity, nr, nt, mdaggm, sigma);
+    if (mode == KQ_MODE_MIXED_FP32 || mode == KQ_MODE_MIXED_FP16) {
+        with_mixed_links(slot, mode, [&](auto fat, auto lng) {
+            apply_t<decltype(fat), decltype(lng), float, double>(fat, lng, g, nrhs, src, dst, parity, nr, nt, mdaggm,
+                                                                 sigma);
+        });
         return KUQCD_SUCCESS;
     }
     set_error("unknown mode");
@@ -174,7 +164,8 @@ static double site_bytes(int mode, bool dbl_gauge, bool long12, int nrhs)
         bl_l = long12 ? 24 + 4 : 36 + 4;
         bv = 24;
     } else if (mode == KQ_MODE_MIXED_FP32 || !dbl_gauge) {
-        bl_f = bl_l = 72;
+        bl_f = 72;
+        bl_l = (long12 && mode == KQ_MODE_MIXED_FP32) ? 48 + 4 : 72;
         bv = 24;
     } else {
         bl_f = bl_l = 144;
@@ -195,19 +186,11 @@ static double time_mode(int slot, int mode, int nrhs, int nr, int nt, int iters,
             return time_dslash<LinkD, LinkD, double>(link_d(c.fat[slot]), link_d(c.lng[slot]), g, nrhs, nr, nt, iters);
         return time_dslash<LinkF, LinkF, float>(link_f(c.fat[slot]), link_f(c.lng[slot]), g, nrhs, nr, nt, iters);
     }
-    if (mode == KQ_MODE_MIXED_FP32) {
-        DerivedFormats &d = derived(slot, true, false);
-        return time_dslash<LinkF, LinkF, float>(as_linkF(d.fat_f, g.Vh), as_linkF(d.lng_f, g.Vh),
-                                                g, nrhs, nr, nt, iters);
-    }
-    DerivedFormats &d = derived(slot, false, true);
-    *long12 = d.long12;
-    LinkH fh = as_linkH(d.fat_h, g.Vh);
-    if (d.long12)
-        return time_dslash<LinkH, LinkH12, float>(fh, as_linkH12(d.lng_h, g.Vh), g,
-                                                  nrhs, nr, nt, iters);
-    return time_dslash<LinkH, LinkH, float>(fh, as_linkH(d.lng_h, g.Vh), g, nrhs, nr,
-                                            nt, iters);
+    double ms = 0.0;
+    *long12 = with_mixed_links(slot, mode, [&](auto fat, auto lng) {
+        ms = time_dslash<decltype(fat), decltype(lng), float>(fat, lng, g, nrhs, nr, nt, iters);
+    });
+    return ms;
 }
 
 std::pair<int, int> tuned_launch(int slot, int mode, int nrhs)

@@ -597,6 +597,7 @@ double vd_sumsq(const double *x, size_t n, hipStream_t st)
     template void pt_weight_grads<LF, LL>(const Geom &, int, LF, LL, const float2 *, const float2 *,                 \
                                           const float2 *, const float2 *, double *, hipStream_t);
 KQ_PT_INST(LinkF, LinkF)
+KQ_PT_INST(LinkF, LinkF12)
 KQ_PT_INST(LinkH, LinkH12)
 KQ_PT_INST(LinkH, LinkH)
 

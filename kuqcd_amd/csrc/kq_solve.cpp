@@ -465,19 +465,8 @@ KuQCDStatus solve_extended(int slot, int nrhs, void **x, void **b, KuQCDInvertPa
     const int mode = (o.mode == KQ_MODE_REFERENCE) ? KQ_MODE_FP64 : o.mode;
     if (mode == KQ_MODE_FP64) {
         run_fp64(P, xdev, rep, res2);
-    } else if (mode == KQ_MODE_MIXED_FP32) {
-        DerivedFormats &d = derived(slot, true, false);
-        run_mixed<LinkF, LinkF>(P, as_linkF(d.fat_f, P.g.Vh), as_linkF(d.lng_f, P.g.Vh), xdev,
-                                rep, res2);
-    } else if (mode == KQ_MODE_MIXED_FP16) {
-        DerivedFormats &d = derived(slot, false, true);
-        LinkH fh = as_linkH(d.fat_h, P.g.Vh);
-        if (d.long12)
-            run_mixed<LinkH, LinkH12>(P, fh, as_linkH12(d.lng_h, P.g.Vh), xdev, rep,
-                                      res2);
-        else
-            run_mixed<LinkH, LinkH>(P, fh, as_linkH(d.lng_h, P.g.Vh), xdev, rep,
-                                    res2);
+    } else if (mode == KQ_MODE_MIXED_FP32 || mode == KQ_MODE_MIXED_FP16) {
+        with_mixed_links(slot, mode, [&](auto fat, auto lng) { run_mixed(P, fat, lng, xdev, rep, res2); });
     } else {
         set_error("unknown mode");
         return KUQCD_ERR_INVALID_PARAM;

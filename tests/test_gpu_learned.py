@@ -126,3 +126,58 @@ def test_identity_network_is_plain_cg(gpu):
     _, rb = gpu.fcg_refine(b, xb, sigma, parity=q, opt=gpu.refine_options(learned=0, rel_tol=1e-10))
     assert ra.converged and rb.converged and ra.fallback == 0
     assert abs(ra.iter - rb.iter) <= max(2, rb.iter // 50)
+
+
+def test_training_reduces_loss_and_refines_held_out(gpu):
+    """Solver-generated pairs on 3 gauge fields, Adam on the GPU kernels, then the flexible CG
+    with the trained network on a held-out field: the training loss falls, the refinement from
+    1e-6 reaches 1e-10 (FP64 true residual) without falling back, and the iteration counts of
+    the learned and the plain refinement are reported."""
+    lat, q = (16, 16, 16, 16), 0
+    mass = 0.005
+    sigma = mass * mass + 1e-4
+    gpu.learned_clear()
+    for s in range(3):
+        _load_weak(gpu, lat, 100 + s)
+        n = gpu.learned_collect(q, sigma, nsrc=1, seed=1000 + s, rel_tol=1e-11, min_level=1e-8)
+    assert n >= 3 * 6
+    rep = gpu.learned_train(steps=150, batch=8, lr=0.02, lr_final=0.002, seed=1, init=1)
+    assert rep.nsamples == n and rep.ngauges == 3
+    assert rep.loss_final < rep.loss_initial
+    gpu.learned_clear()
+    fat, lng = _load_weak(gpu, lat, 500)
+    b = gpu.gen_source(lat, parity=q, seed=7000)
+    x = np.zeros_like(b)
+    st, r6 = gpu.fcg_refine(b, x, sigma, parity=q, opt=gpu.refine_options(learned=0, rel_tol=1e-6))
+    assert st == gpu.KUQCD_SUCCESS and r6.true_rel_res <= 1e-6
+    res = {}
+    for name, lf in (("plain", 0), ("learned", 1)):
+        xx = x.copy()
+        st, r = gpu.fcg_refine(b, xx, sigma, parity=q, opt=gpu.refine_options(learned=lf, rel_tol=1e-10))
+        assert st == gpu.KUQCD_SUCCESS and r.converged == 1
+        assert _true_rel(fat, lng, lat, q, sigma, b, xx) <= 1.05e-10
+        res[name] = r
+    assert res["learned"].fallback == 0 and res["learned"].iter_learned == res["learned"].iter
+    print(f"\nloss {rep.loss_initial:.4f} -> {rep.loss_final:.4f}; refinement 1e-6 -> 1e-10: plain "
+          f"{res['plain'].iter} iterations {res['plain'].seconds:.3f}s, learned {res['learned'].iter} iterations "
+          f"{res['learned'].seconds:.3f}s")
+
+
+def test_mixed_solve_with_learned_refinement(gpu):
+    lat, q = (16, 16, 16, 16), 0
+    mass = 0.01
+    shifts = np.concatenate([[mass * mass + 1e-4], np.logspace(-4, 0, 6)[1:] - 1e-4])
+    _load_weak(gpu, lat, 7)
+    gpu.learned_clear()
+    gpu.learned_collect(q, shifts[0], nsrc=1, seed=5, rel_tol=1e-11, min_level=1e-8)
+    gpu.learned_train(steps=60, batch=4, lr=0.02, seed=2, init=1)
+    gpu.learned_clear()
+    b = gpu.gen_source(lat, parity=q, seed=77)
+    for mode in (2, 3):
+        x = [np.zeros_like(b) for _ in shifts]
+        p = gpu.InvertParam(shifts=shifts, tol=1e-20, max_iter=20000)
+        st, rep = gpu.solve_ex(x, b, p, gpu.default_options(mode=mode, rel_tol=1e-10, learned=1, verbose=0))
+        assert st == gpu.KUQCD_SUCCESS, gpu.last_error()
+        assert rep.learned_used + rep.learned_fallback == 1
+        assert rep.refine_iter[0] > 0
+        assert max(rep.true_rel_res[:len(shifts)]) <= 1e-10
