@@ -87,6 +87,10 @@ def main() -> int:
             assert st == 0, A.last_error()
             print(f"dslash {name}: {r.ms * 1e3:.1f} us, {r.bytes / Vh:.0f} B/site, {r.gbs:.0f} GB/s, "
                   f"nr {r.nr} nt {r.nt}", flush=True)
+        for nr in (2, 4, 8):  # batched FP32 hopping term, links read once for nr right-hand sides
+            st, r = A.bench_dslash(2, nr, nr, 256, 20)
+            assert st == 0, A.last_error()
+            print(f"dslash fp32 x{nr}: {r.ms * 1e3:.1f} us, {r.bytes / Vh:.0f} B/site, {r.gbs:.0f} GB/s", flush=True)
         st, cp = A.bench_copy(1 << 30, 10)
         print(f"copy: 1 GiB read + 1 GiB written per launch, {cp.ms * 1e3:.1f} us, {cp.gbs:.0f} GB/s", flush=True)
         A.kuqcd_gauge_free()

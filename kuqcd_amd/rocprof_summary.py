@@ -1,9 +1,12 @@
 """Short text summaries of rocprofv3 CSV output (kernel statistics and HBM counters).
 
-    python -m kuqcd_amd.rocprof_summary stats <run>_kernel_stats.csv [--top N] [--command "..."]
+    python -m kuqcd_amd.rocprof_summary stats <run>_kernel_stats.csv [--top N] [--filter S] [--command "..."]
+    python -m kuqcd_amd.rocprof_summary phases <run>_kernel_trace.csv [--gap-ms G] [--min-ms M] [--command "..."]
     python -m kuqcd_amd.rocprof_summary traffic --fetch <fetch_run>_counter_collection.csv \
         --write <write_run>_counter_collection.csv --kernel SUBSTRING [--copy-bytes B] [--command "..."]
 
+`phases` splits a kernel trace at idle gaps longer than G ms and, for every phase
+longer than M ms (e.g. one solve), lists its kernels by total time.
 `traffic` reports FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the kernel whose
 name contains SUBSTRING, each scaled by the factor that makes the same counter of
 the plain copy kernel (k_copy16, which reads and writes copy-bytes per launch)
@@ -33,7 +36,12 @@ def _rows(path):
         return header, list(r)
 
 
-def stats(path, top=15):
+def _short(name):
+    n = name.split("(")[0].replace("void ", "").replace("kq::", "")
+    return n if len(n) <= 70 else n[:67] + "..."
+
+
+def stats(path, top=15, filt=""):
     header, rows = _rows(path)
     i_name = _col(header, "Name", "KernelName", "Kernel_Name")
     i_calls = _col(header, "Calls")
@@ -41,12 +49,48 @@ def stats(path, top=15):
     i_avg = _col(header, "AverageNs", "Average_Ns")
     i_pct = _col(header, "Percentage")
     out = [f"{'kernel':<90} {'calls':>7} {'avg us':>10} {'total ms':>10} {'%':>6}"]
+    if filt:
+        rows = [r for r in rows if filt in r[i_name]]
     rows.sort(key=lambda r: -float(r[i_tot]))
     for r in rows[:top]:
         name = r[i_name]
         name = name if len(name) <= 90 else name[:87] + "..."
         out.append(f"{name:<90} {int(r[i_calls]):>7} {float(r[i_avg]) / 1e3:>10.2f} "
                    f"{float(r[i_tot]) / 1e6:>10.3f} {float(r[i_pct]):>6.2f}")
+    return "\n".join(out)
+
+
+def phases(path, gap_ms=0.3, min_ms=20.0, top=20):
+    header, rows = _rows(path)
+    i_name = _col(header, "Kernel_Name", "KernelName", "Name")
+    i_s = _col(header, "Start_Timestamp")
+    i_e = _col(header, "End_Timestamp")
+    rows = sorted(rows, key=lambda r: int(r[i_s]))
+    if not rows:
+        return "(empty trace)"
+    groups, cur = [], [rows[0]]
+    for a, b in zip(rows, rows[1:]):
+        if int(b[i_s]) - int(a[i_e]) > gap_ms * 1e6:
+            groups.append(cur)
+            cur = []
+        cur.append(b)
+    groups.append(cur)
+    t0 = int(rows[0][i_s])
+    out = []
+    for g in groups:
+        s0, e0 = int(g[0][i_s]), int(g[-1][i_e])
+        if (e0 - s0) / 1e6 < min_ms:
+            continue
+        busy = sum(int(r[i_e]) - int(r[i_s]) for r in g)
+        out.append(f"phase at {(s0 - t0) / 1e6:.1f} ms: wall {(e0 - s0) / 1e6:.2f} ms, kernels busy {busy / 1e6:.2f} ms, "
+                   f"{len(g)} dispatches")
+        agg = {}
+        for r in g:
+            k = _short(r[i_name])
+            n, t = agg.get(k, (0, 0))
+            agg[k] = (n + 1, t + int(r[i_e]) - int(r[i_s]))
+        for k, (n, t) in sorted(agg.items(), key=lambda x: -x[1][1])[:top]:
+            out.append(f"    {k:<70} {n:>6} {t / 1e6:>9.2f} ms {t / n / 1e3:>9.1f} us")
     return "\n".join(out)
 
 
@@ -91,7 +135,13 @@ def main(argv=None):
     s = sub.add_parser("stats")
     s.add_argument("csv")
     s.add_argument("--top", type=int, default=15)
+    s.add_argument("--filter", default="")
     s.add_argument("--command", default="")
+    ph = sub.add_parser("phases")
+    ph.add_argument("csv")
+    ph.add_argument("--gap-ms", type=float, default=0.3)
+    ph.add_argument("--min-ms", type=float, default=20.0)
+    ph.add_argument("--command", default="")
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
@@ -101,7 +151,9 @@ def main(argv=None):
     a = ap.parse_args(argv)
     head = f"command: {a.command}\n" if a.command else ""
     if a.cmd == "stats":
-        print(head + stats(Path(a.csv), a.top))
+        print(head + stats(Path(a.csv), a.top, a.filter))
+    elif a.cmd == "phases":
+        print(head + phases(Path(a.csv), a.gap_ms, a.min_ms))
     else:
         print(head + traffic(Path(a.fetch), Path(a.write), a.kernel, a.copy_bytes))
     return 0
