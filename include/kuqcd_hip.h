@@ -139,8 +139,8 @@ KQ_API int kq_memcpy(void *dst, const void *src, size_t bytes);
  *   h_c = sum_t W1[c][t] T_t r  (other parity),  z = W0 r + sum_t T_t sum_c W2[c][t] h_c
  * tap t = mu*4 + 2*(backward) + (hop 3). Weights are in units of the links'
  * root-mean-square row norm nu (per hop), i.e. the kernels use bf16(W / nu_hop).
- * Trained to minimise the energy-norm error left after one step along z = M r,
- *   L(r, e) = 1 - |<z, r>|^2 / (<z, A z> <r, e>),  e = A^{-1} r,
+ * Trained to minimise the error of the correction z = M r in the A^2 norm,
+ *   L(r, e) = |A (z - e)|^2 / |A e|^2,  e = A^{-1} r  (= |r - A M r|^2 / |r|^2 for exact pairs),
  * on (residual, error) pairs that the solver generates. */
 typedef struct {
     float w1[4][16][2]; /* [channel][tap][re, im] */
@@ -177,8 +177,11 @@ typedef struct {
     int batch;        /* samples per step */
     double lr;        /* Adam step size at the start (cosine decay to lr_final) */
     double lr_final;
-    uint64_t seed;    /* sample order */
-    int init;         /* 1: start from kq_ptnet_init_weights(seed, init_scale); 0: from the current weights */
+    uint64_t seed;    /* sample order and initial noise */
+    int init;         /* 0: start from the current weights; 1: kq_ptnet_init_weights(seed, init_scale);
+                         2: channel 0 = the degree-1 polynomial 8/b - 0.9*8/b^2 A of A (b = 1.02 x its
+                         largest eigenvalue by power iteration on the first training gauge), the other
+                         channels random with standard deviation init_scale (0: 0.01) */
     double init_scale;
     int verbose;
 } KqTrainOptions;
@@ -209,7 +212,7 @@ typedef struct {
     int max_iter;
     double delta;     /* 0: 0.1 */
     int chunk;        /* iterations per graph replay (0: 8) */
-    int stall_iter;   /* 0: 2000 */
+    int stall_iter;   /* 0: 300 */
     int verbose;
 } KqRefineOptions;
 typedef struct {

@@ -288,7 +288,7 @@ def learned_clear() -> None:
     L.lib().kq_learned_clear()
 
 
-def learned_train(steps=300, batch=4, lr=0.02, lr_final=0.002, seed=1, init=1, init_scale=0.05,
+def learned_train(steps=200, batch=8, lr=3e-4, lr_final=3e-5, seed=1, init=2, init_scale=0.01,
                   verbose=0) -> L.KqTrainReport:
     o = L.KqTrainOptions(steps=steps, batch=batch, lr=lr, lr_final=lr_final, seed=seed, init=init,
                          init_scale=init_scale, verbose=verbose)

@@ -34,7 +34,7 @@ struct TrainSample {
     int gauge = 0, parity = 0;
     double sigma = 0.0;
     double level = 0.0;  // relative residual of the iterate the pair was taken at
-    DevBuf r, e;         // FP32 SoA, |r| = 1
+    DevBuf r, e, ae;     // FP32 SoA, |r| = 1; ae = A e with the training operator
 };
 
 struct Learned {
@@ -142,6 +142,19 @@ template <class LF, class LL, typename R> struct OpA {
     }
 };
 
+// out = (sigma - D D) x in FP32 with the given links (allocates its own scratch)
+template <class LF, class LL>
+void apply_a_f32(const Geom &g, int q, LF fat, LL lng, double sigma, const float2 *x, float2 *out, int nt,
+                 hipStream_t st)
+{
+    const size_t n3 = (size_t)3 * g.Vh;
+    DevBuf t(sizeof(float2) * n3), sig(sizeof(double)), part(sizeof(double) * dslash_nblk(g.Vh, nt));
+    KQ_HIP_CHECK(hipMemcpyAsync(sig.as<void>(), &sigma, sizeof(double), hipMemcpyHostToDevice, st));
+    const OpA<LF, LL, float> A{g, q, fat, lng, nt};
+    A.apply(x, t.as<float2>(), out, sig.as<double>(), part.as<double>(), nullptr, st);
+    KQ_HIP_CHECK(hipStreamSynchronize(st));
+}
+
 // ---------------------------------------------------------------- flexible CG
 struct FcgCfg {
     double sigma = 0.0;
@@ -149,7 +162,7 @@ struct FcgCfg {
     double delta = 0.1;
     int max_iter = 100000;
     int chunk = 8;
-    int stall = 2000;
+    int stall = 300;
     bool learned = false;
     PtWeightsBF wf{};
     int nt_a = 256, nt_d = 256;
@@ -318,20 +331,22 @@ FcgOut fcg_solve(const Geom &g, int q, AF af, AL al, PF pf, PL pl, LinkD df, Lin
 constexpr int kTrainNT = 128;  // hopping-term block size of the training kernels
 
 struct Work {
-    DevBuf h, z, az, t, gz, gh, sig, part;
+    DevBuf h, z, az, t, d, gz, gh, sig, part;
     size_t n3 = 0;
     int nt = 0;
-    void ensure(uint32_t Vh, int nt)
+    void ensure(uint32_t Vh, int nt_)
     {
         const size_t n = (size_t)3 * Vh;
-        if (n <= n3)
+        if (n <= n3 && nt_ == nt)
             return;
         n3 = n;
+        nt = nt_;
         const size_t fb = sizeof(float2) * n;
         h.alloc(4 * fb);
         z.alloc(fb);
         az.alloc(fb);
         t.alloc(fb);
+        d.alloc(fb);
         gz.alloc(fb);
         gh.alloc(4 * fb);
         sig.alloc(sizeof(double));
@@ -339,15 +354,14 @@ struct Work {
     }
 };
 
-// loss L(r, e) = 1 - |<z, r>|^2 / (<z, A z> <r, e>), z = M r; grad (optional, 2*kPtParams
-// doubles): dL/dconj(w) for the master weights
+// loss L = |A(z - e)|^2 / |A e|^2 with z = M r, for a pair (r, e) with A e = ae given:
+// the error of the correction z in the A^2 norm (for a solver pair, r = A e and L is
+// the residual one application of M leaves). grad (optional, 2*kPtParams doubles):
+// dL/dconj(w) of the master weights, from dL/dconj(z) = A (A z - A e) / |A e|^2.
 template <class LF, class LL>
 double sample_loss(const Geom &g, int q, LF fat, LL lng, const PtNorms &nu, const PtWeights &w, double sigma,
-                   const float2 *r, const float2 *e, Work &W, int nt, double *grad, hipStream_t st)
+                   const float2 *r, const float2 *ae, Work &W, int nt, double *grad, hipStream_t st)
 {
-    if (W.nt != nt)
-        W.n3 = 0;
-    W.nt = nt;
     W.ensure(g.Vh, nt);
     const size_t n3 = (size_t)3 * g.Vh;
     pt_forward<LF, LL>(g, q, fat, lng, pt_pack(w, nu), r, W.h.as<float2>(), W.z.as<float2>(), st);
@@ -355,23 +369,24 @@ double sample_loss(const Geom &g, int q, LF fat, LL lng, const PtNorms &nu, cons
     const OpA<LF, LL, float> A{g, q, fat, lng, nt};
     A.apply(W.z.as<float2>(), W.t.as<float2>(), W.az.as<float2>(), W.sig.as<double>(), W.part.as<double>(), nullptr,
             st);
-    const float2 *xs[3] = {W.z.as<float2>(), W.z.as<float2>(), r};
-    const float2 *ys[3] = {r, W.az.as<float2>(), e};
-    double2 d[3];
-    vf_dots(xs, ys, 3, n3, d, st);
-    const double a2 = d[0].x * d[0].x + d[0].y * d[0].y, Nz = d[1].x, Ne = d[2].x;
+    vf_lincomb(W.d.as<float2>(), W.az.as<float2>(), mk<float2>(1.f, 0.f), ae, mk<float2>(-1.f, 0.f), n3, st);
+    const float2 *xs[2] = {W.d.as<float2>(), ae};
+    const float2 *ys[2] = {W.d.as<float2>(), ae};
+    double2 dd[2];
+    vf_dots(xs, ys, 2, n3, dd, st);
+    const double num = dd[0].x, den = dd[1].x;
     if (grad)
         std::fill(grad, grad + 2 * kPtParams, 0.0);
-    if (!(Nz > 0.0) || !(Ne > 0.0))
-        return 1.0;
-    const double loss = 1.0 - a2 / (Nz * Ne);
+    if (!(den > 0.0))
+        return 0.0;
+    const double loss = num / den;
     if (!grad)
         return loss;
-    // dL/dconj(z) = (|a|^2 A z - conj(a) <z,Az> r) / (<z,Az>^2 <r,e>)
-    const double c1 = a2 / (Nz * Nz * Ne);
-    const double c2r = -d[0].x / (Nz * Ne), c2i = d[0].y / (Nz * Ne);
-    vf_lincomb(W.gz.as<float2>(), W.az.as<float2>(), mk<float2>((float)c1, 0.f), r,
-               mk<float2>((float)c2r, (float)c2i), n3, st);
+    A.apply(W.d.as<float2>(), W.t.as<float2>(), W.gz.as<float2>(), W.sig.as<double>(), W.part.as<double>(), nullptr,
+            st);
+    const float inv = (float)(1.0 / den);
+    vf_lincomb(W.gz.as<float2>(), W.gz.as<float2>(), mk<float2>(inv, 0.f), W.gz.as<float2>(), mk<float2>(0.f, 0.f),
+               n3, st);
     pt_backward_hidden<LF, LL>(g, q, fat, lng, pt_pack_backward(w, nu), W.gz.as<float2>(), W.gh.as<float2>(), st);
     pt_weight_grads<LF, LL>(g, q, fat, lng, r, W.h.as<float2>(), W.gz.as<float2>(), W.gh.as<float2>(), grad, st);
     // w_eff = w / nu_hop for the tap weights
@@ -379,9 +394,9 @@ double sample_loss(const Geom &g, int q, LF fat, LL lng, const PtNorms &nu, cons
         for (int c = 0; c < kPtHidden; ++c)
             for (int t = 0; t < kPtTaps; ++t) {
                 const int j = l * kPtHidden * kPtTaps + c * kPtTaps + t;
-                const double s = 1.0 / nu.nu[pt_tap_hop(t) == 3 ? 1 : 0];
-                grad[2 * j] *= s;
-                grad[2 * j + 1] *= s;
+                const double sc = 1.0 / nu.nu[pt_tap_hop(t) == 3 ? 1 : 0];
+                grad[2 * j] *= sc;
+                grad[2 * j + 1] *= sc;
             }
     return loss;
 }
@@ -390,7 +405,7 @@ double train_sample_loss(const TrainSample &s, const PtWeights &w, Work &W, doub
 {
     const TrainGauge &G = *learned().gauges[s.gauge];
     const LinkF fat = as_linkF(G.fat, G.g.Vh), lng = as_linkF(G.lng, G.g.Vh);
-    return sample_loss<LinkF, LinkF>(G.g, s.parity, fat, lng, G.nu, w, s.sigma, s.r.as<float2>(), s.e.as<float2>(),
+    return sample_loss<LinkF, LinkF>(G.g, s.parity, fat, lng, G.nu, w, s.sigma, s.r.as<float2>(), s.ae.as<float2>(),
                                      W, kTrainNT, grad, st);
 }
 
@@ -412,6 +427,57 @@ void init_weights(PtWeights &w, uint64_t seed, double scale)
             w.w2[c][t] = mk<float2>((float)(scale * rng.normal()), (float)(scale * rng.normal()));
         }
     w.w0 = mk<float2>(1.f, 0.f);
+}
+
+// Largest eigenvalue of A = sigma - D D on the first training gauge (power iteration, FP32).
+double lambda_max_train(int iters, hipStream_t st)
+{
+    const Learned &L = learned();
+    const TrainGauge &G = *L.gauges[L.samples[0]->gauge];
+    const int q = L.samples[0]->parity;
+    const double sigma = L.samples[0]->sigma;
+    const size_t n3 = (size_t)3 * G.g.Vh;
+    std::vector<float2> h(n3);
+    Rng rng{0x5eedull};
+    for (auto &v : h)
+        v = mk<float2>((float)rng.normal(), (float)rng.normal());
+    DevBuf y(sizeof(float2) * n3), ay(sizeof(float2) * n3);
+    KQ_HIP_CHECK(hipMemcpyAsync(y.as<void>(), h.data(), sizeof(float2) * n3, hipMemcpyHostToDevice, st));
+    const LinkF fat = as_linkF(G.fat, G.g.Vh), lng = as_linkF(G.lng, G.g.Vh);
+    double lam = 0.0;
+    for (int i = 0; i < iters; ++i) {
+        apply_a_f32(G.g, q, fat, lng, sigma, y.as<float2>(), ay.as<float2>(), kTrainNT, st);
+        const float2 *xs[2] = {y.as<float2>(), ay.as<float2>()};
+        const float2 *ys[2] = {y.as<float2>(), ay.as<float2>()};
+        double2 d[2];
+        vf_dots(xs, ys, 2, n3, d, st);
+        lam = std::sqrt(d[1].x / d[0].x);
+        const float inv = (float)(1.0 / std::sqrt(d[1].x));
+        vf_lincomb(y.as<float2>(), ay.as<float2>(), mk<float2>(inv, 0.f), ay.as<float2>(), mk<float2>(0.f, 0.f), n3,
+                   st);
+    }
+    return lam;
+}
+
+// Start from the degree-1 polynomial of A that a Chebyshev bound suggests,
+// M = 8/beta - 0.9 * 8/beta^2 * A (beta = 1.02 lambda_max, positive on the spectrum),
+// written in channel 0 (h_0 = D r, then D h_0 = D D r = sigma - A); the other
+// channels get small random weights.
+void init_chebyshev(PtWeights &w, uint64_t seed, double noise, hipStream_t st)
+{
+    const Learned &L = learned();
+    const PtNorms &nu = L.gauges[L.samples[0]->gauge]->nu;
+    const double sigma = L.samples[0]->sigma;
+    const double beta = 1.02 * lambda_max_train(40, st);
+    const double c = 0.9 * 8.0 / (beta * beta);
+    init_weights(w, seed, noise);
+    for (int t = 0; t < kPtTaps; ++t) {
+        const double d = (t & 2) ? -1.0 : 1.0;  // sign of tap t in the hopping term
+        const double n = nu.nu[pt_tap_hop(t) == 3 ? 1 : 0];
+        w.w1[0][t] = mk<float2>((float)(d * n), 0.f);
+        w.w2[0][t] = mk<float2>((float)(c * d * n), 0.f);
+    }
+    w.w0 = mk<float2>((float)(8.0 / beta - c * sigma), 0.f);
 }
 
 int tuned_nt(int slot, int mode)
@@ -464,7 +530,7 @@ RefineResult fcg_refine_device(int slot, int parity, double sigma, const double 
     cfg.delta = o.delta > 0 ? o.delta : 0.1;
     cfg.max_iter = o.max_iter > 0 ? o.max_iter : 100000;
     cfg.chunk = o.chunk > 0 ? o.chunk : 8;
-    cfg.stall = o.stall_iter > 0 ? o.stall_iter : 2000;
+    cfg.stall = o.stall_iter > 0 ? o.stall_iter : 300;
     cfg.learned = use_net;
     cfg.verbose = o.verbose;
     cfg.nt_a = tuned_nt(slot, amode);
@@ -592,14 +658,15 @@ KuQCDStatus kq_ptnet_loss_grad(const KqPtWeights *w, int parity, double sigma, c
         const int slot = slot_or_throw();
         const Geom g = slot_geom(slot);
         const size_t n3 = (size_t)3 * g.Vh;
-        DevBuf vr(sizeof(float2) * n3), ve(sizeof(float2) * n3), stage;
+        DevBuf vr(sizeof(float2) * n3), ve(sizeof(float2) * n3), vae(sizeof(float2) * n3), stage;
         upload_vec<double, float>(r, Vec<float>{vr.as<float2>(), g.Vh, n3}, 0, stage, c.stream);
         upload_vec<double, float>(e, Vec<float>{ve.as<float2>(), g.Vh, n3}, 0, stage, c.stream);
         DerivedFormats &d = derived(slot, true, false);
+        const LinkF fat = as_linkF(d.fat_f, g.Vh), lng = as_linkF(d.lng_f, g.Vh);
+        apply_a_f32(g, parity, fat, lng, sigma, ve.as<float2>(), vae.as<float2>(), kTrainNT, c.stream);
         Work W;
-        *loss = sample_loss<LinkF, LinkF>(g, parity, as_linkF(d.fat_f, g.Vh), as_linkF(d.lng_f, g.Vh),
-                                          slot_norms(slot), from_abi(*w), sigma, vr.as<float2>(), ve.as<float2>(), W,
-                                          kTrainNT, grad, c.stream);
+        *loss = sample_loss<LinkF, LinkF>(g, parity, fat, lng, slot_norms(slot), from_abi(*w), sigma, vr.as<float2>(),
+                                          vae.as<float2>(), W, kTrainNT, grad, c.stream);
         return KUQCD_SUCCESS;
     });
 }
@@ -680,6 +747,8 @@ KuQCDStatus kq_learned_collect(int parity, double sigma, int nsrc, uint64_t seed
                 const double scale = 1.0 / (lv[k] * std::sqrt(b2));
                 fcg_to_f32(rs[k].as<double2>(), nullptr, scale, s->r.as<float2>(), n3, c.stream);
                 fcg_to_f32(x64.as<double2>(), xs[k].as<double2>(), scale, s->e.as<float2>(), n3, c.stream);
+                s->ae.alloc(sizeof(float2) * n3);
+                apply_a_f32(g, parity, af, al, sigma, s->e.as<float2>(), s->ae.as<float2>(), kTrainNT, c.stream);
                 L.samples.push_back(std::move(s));
             }
             KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
@@ -734,8 +803,11 @@ KuQCDStatus kq_learned_train(const KqTrainOptions *o, KqTrainReport *rep)
             throw std::runtime_error("the training set is empty: call kq_learned_collect first");
         const auto t0 = clk::now();
         PtWeights w = L.w;
-        if (o->init || !L.have)
-            init_weights(w, o->seed, o->init_scale > 0 ? o->init_scale : 0.05);
+        const double noise = o->init_scale > 0 ? o->init_scale : 0.01;
+        if (o->init == 1)
+            init_weights(w, o->seed, noise);
+        else if (o->init == 2 || !L.have)
+            init_chebyshev(w, o->seed, noise, c.stream);
         Work W;
         KqTrainReport R;
         std::memset(&R, 0, sizeof R);
@@ -804,7 +876,7 @@ void kq_default_refine_options(KqRefineOptions *o)
     o->max_iter = 100000;
     o->delta = 0.1;
     o->chunk = 8;
-    o->stall_iter = 2000;
+    o->stall_iter = 300;
     o->verbose = 0;
 }
 

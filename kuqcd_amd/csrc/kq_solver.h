@@ -357,9 +357,10 @@ private:
         KQ_HIP_CHECK(hipStreamSynchronize(st_));
     }
 
-    // ax64 = A_0 x_hi[.][0] for the listed systems (FP64 links)
-    void apply_a64_x0()
+    // ax64 = A_0 x_hi[.][0] for the n listed systems (FP64 links, read once for up to 4 systems)
+    void apply_a64_x0(int n)
     {
+        const int nr = n >= 4 ? 4 : n >= 2 ? 2 : 1;
         DslashArgs<LinkD, LinkD, double> a{};
         a.g = g_;
         a.fat = fat64_;
@@ -373,14 +374,14 @@ private:
         a.dst_parity = 1 - parity_;
         a.src = x0;
         a.dst = t;
-        launch_dslash(a, EPI_STORE, 1, 256, st_);
+        launch_dslash(a, EPI_STORE, nr, 256, st_);
         a.dst_parity = parity_;
         a.src = t;
         a.dst = ax;
         a.aux = x0;
         a.sigma = sigma_.as<double>();
         a.partial = part_a_.as<double>();
-        launch_dslash(a, EPI_AXMY_DOT, 1, 256, st_);
+        launch_dslash(a, EPI_AXMY_DOT, nr, 256, st_);
     }
 
     void start_pending()
@@ -400,7 +401,7 @@ private:
         if (reliable_) {
             if (!pending_x0_.empty()) {
                 set_list(pending_x0_);
-                apply_a64_x0();
+                apply_a64_x0((int)pending_x0_.size());
             }
             set_list(pending_);
             rel_residual<VR>(b_hi_.as<double>(), ax64_.as<double>(), vec(r_), list(), list_n(), max_sys_,
@@ -429,7 +430,7 @@ private:
         }
         if (!rel.empty()) {
             set_list(rel);
-            apply_a64_x0();
+            apply_a64_x0((int)rel.size());
             rel_residual<VR>(b_hi_.as<double>(), ax64_.as<double>(), vec(r_), list(), list_n(), max_sys_,
                              part_b_.as<double>(), nblk_, st_);
             rel_set_rr<S>(state_.as<CgmState<S>>(), list(), list_n(), max_sys_, part_b_.as<double>(), nblk_, st_);

@@ -50,9 +50,9 @@ def main(argv=None) -> int:
     ap.add_argument("--train", type=int, default=16)
     ap.add_argument("--test", type=int, default=8)
     ap.add_argument("--nsrc", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--lr", type=float, default=0.02)
+    ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--mode", type=int, default=A.KQ_MODE_MIXED_FP32, help="link format of A in the refinement")
     ap.add_argument("--save", default="")
     a = ap.parse_args(argv)
@@ -67,7 +67,7 @@ def main(argv=None) -> int:
         n = A.learned_collect(0, sigma, nsrc=a.nsrc, seed=1000 + s, rel_tol=1e-11, min_level=1e-8)
         print(f"train seed {s}: {n} pairs so far", file=sys.stderr, flush=True)
     t_collect = time.time() - t0
-    rep = A.learned_train(steps=a.steps, batch=a.batch, lr=a.lr, lr_final=a.lr / 10, seed=1, init=1)
+    rep = A.learned_train(steps=a.steps, batch=a.batch, lr=a.lr, lr_final=a.lr / 10, seed=1, init=2)
     w = A.ptnet_get_weights()
     if a.save:
         save_weights(w, a.save)

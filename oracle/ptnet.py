@@ -190,7 +190,9 @@ def apply_A(fat, lng, lat, q: int, sigma: float, z: np.ndarray) -> np.ndarray:
 
 
 def loss_grad(fat, lng, lat, q: int, sigma: float, W1, W2, w0, r, e, nu=None, round_bf16=True, want_grad=True):
-    """L = 1 - |<z,r>|^2 / (<z,Az> <r,e>), z = M r, and dL/dconj of the master weights (complex [129])."""
+    """L = |A(z - e)|^2 / |A e|^2 with z = M r (the error of the correction z in the A^2 norm; for a
+    solver pair r = A e it is the residual left by one application of M), and dL/dconj of the master
+    weights (complex [129])."""
     U1, U3 = links(fat, lng, lat)
     G = Geometry(lat)
     nu = norms(U1, U3) if nu is None else nu
@@ -199,13 +201,13 @@ def loss_grad(fat, lng, lat, q: int, sigma: float, W1, W2, w0, r, e, nu=None, ro
     e = spinor(e) if np.asarray(e).dtype != np.complex128 else e
     h, z = forward(G, U1, U3, W1e, W2e, w0e, q, r)
     Az = apply_A(fat, lng, lat, q, sigma, z)
-    a = np.vdot(z, r)
-    Nz = np.vdot(z, Az).real
-    Ne = np.vdot(r, e).real
-    L = 1.0 - abs(a) ** 2 / (Nz * Ne)
+    Ae = apply_A(fat, lng, lat, q, sigma, e)
+    d = Az - Ae
+    n2 = np.vdot(Ae, Ae).real
+    L = np.vdot(d, d).real / n2
     if not want_grad:
         return L, None
-    g = (abs(a) ** 2 * Az - np.conj(a) * Nz * r) / (Nz * Nz * Ne)
+    g = apply_A(fat, lng, lat, q, sigma, d) / n2
     gW1, gW2, gw0 = backward(G, U1, U3, W2e, q, r, h, g)
     s = np.array([1.0 / nu[1] if (t & 1) else 1.0 / nu[0] for t in range(NT)])
     return L, np.concatenate([(gW1 * s).ravel(), (gW2 * s).ravel(), [gw0]])

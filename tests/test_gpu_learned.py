@@ -69,8 +69,8 @@ def test_loss_and_gradient_match_oracle(gpu, lat, q):
     sigma = 0.02
     L_o, g_o = P.loss_grad(fat, lng, lat, q, sigma, W1, W2, w0, r, e, nu=nu)
     L, g = gpu.ptnet_loss_grad(w, r, e, sigma, parity=q)
-    assert 0.0 < L_o < 1.0
-    assert abs(L - L_o) < 1e-5
+    assert L_o > 0.0
+    assert abs(L - L_o) < 1e-5 * L_o
     assert np.linalg.norm(g - g_o) < 1e-4 * np.linalg.norm(g_o)
     # per block: hop-1 taps, hop-3 taps (scaled by 1/nu_long), skip weight
     for sel in (slice(0, 128, 2), slice(1, 128, 2), slice(128, 129)):
@@ -141,7 +141,7 @@ def test_training_reduces_loss_and_refines_held_out(gpu):
         _load_weak(gpu, lat, 100 + s)
         n = gpu.learned_collect(q, sigma, nsrc=1, seed=1000 + s, rel_tol=1e-11, min_level=1e-8)
     assert n >= 3 * 6
-    rep = gpu.learned_train(steps=150, batch=8, lr=0.02, lr_final=0.002, seed=1, init=1)
+    rep = gpu.learned_train(steps=150, batch=8, seed=1)
     assert rep.nsamples == n and rep.ngauges == 3
     assert rep.loss_final < rep.loss_initial
     gpu.learned_clear()
@@ -170,7 +170,7 @@ def test_mixed_solve_with_learned_refinement(gpu):
     _load_weak(gpu, lat, 7)
     gpu.learned_clear()
     gpu.learned_collect(q, shifts[0], nsrc=1, seed=5, rel_tol=1e-11, min_level=1e-8)
-    gpu.learned_train(steps=60, batch=4, lr=0.02, seed=2, init=1)
+    gpu.learned_train(steps=60, batch=4, seed=2)
     gpu.learned_clear()
     b = gpu.gen_source(lat, parity=q, seed=77)
     for mode in (2, 3):
