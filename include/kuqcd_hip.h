@@ -42,7 +42,8 @@ typedef enum {
     KQ_MODE_REFERENCE = 0,  /* the reference's method in the loaded gauge precision */
     KQ_MODE_FP64 = 1,       /* plain FP64 multishift CG (double gauge) */
     KQ_MODE_MIXED_FP32 = 2, /* FP32 links/vectors, FP64 reliable updates + per-shift refinement */
-    KQ_MODE_MIXED_FP16 = 3  /* FP16 links (+ per-link FP32 scale, 12-real long links), FP32 vectors */
+    KQ_MODE_MIXED_FP16 = 3  /* 16-bit links: fixed-point int16 entries with a per-link FP32 scale
+                               (12-real long links), FP32 vectors */
 } KqMode;
 
 typedef struct {
@@ -57,7 +58,7 @@ typedef struct {
     int verbose;           /* 0 silent, 1 iteration summary (reference print), 2 details */
     double fork_tol;       /* mixed modes: a shift k >= 1 leaves the multishift and continues as its own
                               FP64-reliable CG once its iterated relative residual is below fork_tol
-                              (0: automatic, 1e-5 for FP32 and 1e-3 for FP16 links; < 0: never) */
+                              (0: automatic, 1e-7; < 0: never) */
     double learned_tol;    /* learned = 1: relative residual at which shift 0 leaves the multishift for
                               the flexible-CG refinement (0: 1e-6) */
 } KqSolveOptions;

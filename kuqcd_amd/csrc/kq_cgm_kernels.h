@@ -26,7 +26,8 @@ template <typename S> struct CgmState {
     double tolk[kMaxShift];   // absolute squared-residual target per shift (inverter_cgm.cpp:72-73)
     double r2max;             // |r|^2 at the last reliable update (mixed precision)
     double pause2;            // pause when |r|^2 < pause2 * r2max (0: never)
-    int nshift, maxTerm, maxTermUpd, iter, maxIter, stop, run, maxTermFold, pause, pad;
+    int nshift, maxTerm, maxTermUpd, iter, maxIter, stop, run, maxTermFold, pause;
+    int discard;  // set by the host for one fold: drop x_lo instead of adding it (diverged iteration)
 };
 
 struct CgmLaunch {
@@ -67,6 +68,11 @@ void rel_residual(const double *b, const double *ax, Vec<VR> r_lo, const int *li
 template <typename S>
 void rel_set_rr(CgmState<S> *st, const int *list, const int *n, int nsys, const double *part, uint32_t nblk,
                 hipStream_t stream);
+// restart of listed systems that have a single live shift (after a reliable update
+// has set r = b - A x): the recurrence starts afresh from r (p = r is copied by
+// copy_r_to_p) while iter, maxTerm and the folded solutions are kept
+template <typename S>
+void cgm_restart_list(CgmState<S> *st, const int *list, const int *n, int nsys, hipStream_t stream);
 // x_hi[k] += x_lo[k]; x_lo[k] = 0 for k < st[sys].maxTermFold
 template <typename S, typename VR>
 void rel_fold(CgmState<S> *st, double *x_hi, VR *x_lo, const size_t *off, uint32_t Vh, const int *list, const int *n,

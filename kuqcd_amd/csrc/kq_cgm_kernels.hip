@@ -251,6 +251,7 @@ __global__ void __launch_bounds__(256) k_cgm_init_list(CgmState<S> *st, const in
     c.r2max = rr;
     c.iter = 0;
     c.pause = 0;
+    c.discard = 0;
     // the reference's do-while runs at least one iteration even for max_iter <= 0
     c.stop = (c.nshift <= 0) ? 1 : 0;
     c.run = 0;
@@ -351,6 +352,7 @@ __global__ void __launch_bounds__(256) k_rel_set_rr(CgmState<S> *st, const int *
     if (threadIdx.x == 0) {
         st[rh].rr = (S)rr;
         st[rh].r2max = rr;
+        st[rh].discard = 0;
     }
 }
 
@@ -359,6 +361,29 @@ void rel_set_rr(CgmState<S> *st, const int *list, const int *n, int nsys, const 
                 hipStream_t stream)
 {
     hipLaunchKernelGGL(k_rel_set_rr<S>, dim3(nsys), dim3(256), 0, stream, st, list, n, part, nblk);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+template <typename S> __global__ void k_cgm_restart_list(CgmState<S> *st, const int *list, const int *n)
+{
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= *n)
+        return;
+    CgmState<S> &c = st[list[slot]];
+    c.Bm1 = (S)1.0;
+    for (int k = 0; k < kMaxShift; ++k) {
+        c.a[k] = (S)0.0;
+        c.B[k] = (S)1.0;
+        c.Z[k] = (S)1.0;
+        c.Zm1[k] = (S)1.0;
+    }
+    c.r2max = (double)c.rr;
+    c.pause = 0;
+}
+
+template <typename S> void cgm_restart_list(CgmState<S> *st, const int *list, const int *n, int nsys, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_cgm_restart_list<S>, dim3((nsys + 63) / 64), dim3(64), 0, stream, st, list, n);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
@@ -376,15 +401,18 @@ __global__ void __launch_bounds__(kBlasNT) k_rel_fold(const CgmState<S> *st, dou
     if (i >= n3)
         return;
     const int m = st[rh].maxTermFold;
+    const bool keep = st[rh].discard == 0;
     for (int k = 0; k < m; ++k) {
         const size_t e = off[rh] + (size_t)k * n3 + i;
         V2 *xl = reinterpret_cast<V2 *>(x_lo) + e;
         double2 *xh = reinterpret_cast<double2 *>(x_hi) + e;
-        const V2 l = *xl;
-        double2 h = *xh;
-        h.x += (double)l.x;
-        h.y += (double)l.y;
-        *xh = h;
+        if (keep) {
+            const V2 l = *xl;
+            double2 h = *xh;
+            h.x += (double)l.x;
+            h.y += (double)l.y;
+            *xh = h;
+        }
         V2 z;
         z.x = 0;
         z.y = 0;
@@ -489,7 +517,8 @@ void reduce_partials(const double *part, uint32_t nblk, int nrhs, double *out, h
     template void cgm_init_list<S>(CgmState<S> *, const int *, const int *, int, const double *, uint32_t,    \
                                    hipStream_t);                                                              \
     template void rel_set_rr<S>(CgmState<S> *, const int *, const int *, int, const double *, uint32_t,       \
-                                hipStream_t);
+                                hipStream_t);                                                              \
+    template void cgm_restart_list<S>(CgmState<S> *, const int *, const int *, int, hipStream_t);
 KQ_INST_S(double)
 KQ_INST_S(float)
 

@@ -30,9 +30,12 @@ __device__ inline kq_v4u ld_nt(const kq_v4u *p) { return __builtin_nontemporal_l
 __device__ inline kq_v2u ld_nt(const kq_v2u *p) { return __builtin_nontemporal_load(p); }
 __device__ inline float ld_nt(const float *p) { return __builtin_nontemporal_load(p); }
 
-__device__ inline float2 h2f(unsigned int u)
+// two 16-bit fixed-point reals (low half: real part) -> floats in units of the
+// stored scale: q / 32767 for |q| <= 32767
+__device__ inline float2 q2f(unsigned int u)
 {
-    return __half22float2(*reinterpret_cast<const __half2 *>(&u));
+    const float re = (float)(short)(u & 0xffffu), im = (float)(short)(u >> 16);
+    return mk<float2>(re, im);
 }
 
 // ------------------------------------------------------------ links
@@ -85,7 +88,9 @@ struct LinkF {  // FP32, 18 reals: 4 chunks of 16 B (complex 0..7) + 1 of 8 B (c
     }
 };
 
-// FP16, 18 reals, one FP32 scale per link: U = scale * half(U/scale).
+// 16-bit links, 18 reals: each real is a 16-bit fixed-point integer q with
+// |q| <= 32767 and U = a * q, where a = max|entry| / 32767 is one FP32 scale per
+// link (the lattice "half" format: 15 bits per entry relative to the largest).
 // Chunks: 16 B (complex 0..3), 16 B (complex 4..7), 8 B (complex 8 + scale).
 struct LinkH {
     using R = float;
@@ -104,15 +109,15 @@ struct LinkH {
         const unsigned int h[9] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x};
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            const float2 f = h2f(h[k]);
+            const float2 f = q2f(h[k]);
             u[k] = mk<float2>(f.x * a, f.y * a);
         }
     }
 };
 
-// FP16, 12 reals (rows 0 and 1 of an SU(3) matrix V), one signed FP32 scale b
-// per link: U = b * V with row 2 of V = conj(row0 x row1).
-// Chunks: 16 B (complex 0..3), 8 B (complex 4, 5), 4 B (scale).
+// 16-bit links, 12 reals (rows 0 and 1 of an SU(3) matrix V as fixed point
+// q / 32767), one signed FP32 scale b per link: U = b * V with row 2 of V =
+// conj(row0 x row1). Chunks: 16 B (complex 0..3), 8 B (complex 4, 5), 4 B (scale).
 struct LinkH12 {
     using R = float;
     static constexpr int kBytes = 28;
@@ -126,13 +131,17 @@ struct LinkH12 {
         const kq_v4u c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
         const kq_v2u c1 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 1024) + l);
         const float a = ld_nt(reinterpret_cast<const float *>(b + 1536) + l);
+        constexpr float kQ = 1.0f / 32767.0f;
         float2 v[6];
-        v[0] = h2f(c0.x);
-        v[1] = h2f(c0.y);
-        v[2] = h2f(c0.z);
-        v[3] = h2f(c0.w);
-        v[4] = h2f(c1.x);
-        v[5] = h2f(c1.y);
+        v[0] = q2f(c0.x);
+        v[1] = q2f(c0.y);
+        v[2] = q2f(c0.z);
+        v[3] = q2f(c0.w);
+        v[4] = q2f(c1.x);
+        v[5] = q2f(c1.y);
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            v[k] = mk<float2>(v[k].x * kQ, v[k].y * kQ);
         // row2 = conj(row0 x row1)
         float2 w[3];
 #pragma unroll

@@ -22,10 +22,10 @@ void gauge_host_to_soa(const HR *raw, size_t i0, size_t n, c2_t<LR> *dst, uint32
 // FP64 -> LinkD; FP64 or FP32 -> LinkF (rounding to nearest)
 void canon_to_linkD(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st);
 template <typename CR> void canon_to_linkF(const c2_t<CR> *canon, char *dst, uint32_t Vh, hipStream_t st);
-// FP64 -> LinkH: FP16 entries with one FP32 scale per link (max |entry|)
+// FP64 -> LinkH: 16-bit fixed-point entries with one FP32 scale per link (max |entry| / 32767)
 void canon_to_linkH(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st);
 // FP64 links of the form b * V (V in SU(3), b real) -> LinkH12: rows 0,1 of V in
-// FP16 and b in FP32. fail[0] counts links that are not of that form to 1e-6.
+// 16-bit fixed point and b in FP32. fail[0] counts links that are not of that form to 1e-6.
 void canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);
 // LinkF12 from canonical double links; *fail counts links that are not b * SU(3)
 void canon_to_linkF12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);

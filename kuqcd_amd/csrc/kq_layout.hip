@@ -90,10 +90,12 @@ template <typename CR> void canon_to_linkF(const c2_t<CR> *canon, char *dst, uin
     KQ_HIP_CHECK(hipGetLastError());
 }
 
-__device__ inline unsigned int pack_h2(float a, float b)
+// two reals in [-1, 1] -> 16-bit fixed point q = round(32767 x), real part in the low half
+__device__ inline unsigned int pack_q2(double a, double b)
 {
-    __half2 h = __floats2half2_rn(a, b);
-    return *reinterpret_cast<unsigned int *>(&h);
+    const int qa = (int)rint(fmin(fmax(a, -1.0), 1.0) * 32767.0);
+    const int qb = (int)rint(fmin(fmax(b, -1.0), 1.0) * 32767.0);
+    return ((unsigned int)(unsigned short)(short)qa) | ((unsigned int)(unsigned short)(short)qb << 16);
 }
 
 __global__ void k_canon_to_linkH(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64)
@@ -108,15 +110,15 @@ __global__ void k_canon_to_linkH(const double2 *canon, char *dst, uint32_t Vh, u
         m[k] = canon[((size_t)qm * 9 + k) * Vh + s];
         amax = fmax(amax, fmax(fabs(m[k].x), fabs(m[k].y)));
     }
-    const float a = amax > 0.0 ? (float)amax : 1.0f;
+    const double a = amax > 0.0 ? amax : 1.0;
     unsigned int h[9];
     for (int k = 0; k < 9; ++k)
-        h[k] = pack_h2((float)(m[k].x / a), (float)(m[k].y / a));
+        h[k] = pack_q2(m[k].x / a, m[k].y / a);
     char *b = dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkH::kBlock);
     const uint32_t l = s & 63;
     reinterpret_cast<uint4 *>(b)[l] = make_uint4(h[0], h[1], h[2], h[3]);
     reinterpret_cast<uint4 *>(b + 1024)[l] = make_uint4(h[4], h[5], h[6], h[7]);
-    reinterpret_cast<uint2 *>(b + 2048)[l] = make_uint2(h[8], __float_as_uint(a));
+    reinterpret_cast<uint2 *>(b + 2048)[l] = make_uint2(h[8], __float_as_uint((float)(a / 32767.0)));
 }
 
 void canon_to_linkH(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st)
@@ -184,7 +186,7 @@ __global__ void k_canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh,
         atomicAdd(fail, 1);
     unsigned int h[6];
     for (int k = 0; k < 6; ++k)
-        h[k] = pack_h2((float)v[k].x, (float)v[k].y);
+        h[k] = pack_q2(v[k].x, v[k].y);
     char *b = dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkH12::kBlock);
     const uint32_t l = s & 63;
     reinterpret_cast<uint4 *>(b)[l] = make_uint4(h[0], h[1], h[2], h[3]);

@@ -212,7 +212,7 @@ std::pair<int, int> tuned_launch(int slot, int mode, int nrhs)
     bool l12 = false;
     for (int nr : nrs)
         for (int nt : {64, 128, 256}) {
-            const double ms = time_mode(slot, mode, nrhs, nr, nt, 3, &l12);
+            const double ms = time_mode(slot, mode, nrhs, nr, nt, 8, &l12);
             if (ms < best) {
                 best = ms;
                 pick = {nr, nt};

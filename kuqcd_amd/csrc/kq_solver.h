@@ -257,6 +257,7 @@ public:
                 KQ_HIP_CHECK(hipGraphLaunch(gx, st_));
             download_states();
             std::vector<int> fold, rel;
+            std::vector<double> rr_iter(nsys_, 0.0);
             std::vector<std::pair<int, int>> drops;
             bool live = false;
             for (int sys = 0; sys < nsys_; ++sys) {
@@ -279,7 +280,13 @@ public:
                     continue;
                 }
                 live = true;
-                const bool need_rel = reliable_ && (c.pause || (double)c.rr < delta_ * delta_ * c.r2max);
+                // a single-shift iteration whose residual grows far above the last true one has
+                // lost its recurrence to the sloppy operator: update and restart it
+                const bool diverging = reliable_ && c.maxTerm <= 1 && !((double)c.rr < 1e2 * c.r2max);
+                const bool need_rel =
+                    reliable_ && (c.pause || (double)c.rr < delta_ * delta_ * c.r2max || diverging);
+                rr_iter[sys] = diverging ? 0.0 : (double)c.rr;
+                c.discard = diverging ? 1 : 0;  // the updates since the last fold are not trusted
                 if (reliable_ && (need_rel || dropped)) {
                     c.maxTermFold = last_fold_[sys];
                     fold.push_back(sys);
@@ -291,6 +298,23 @@ public:
             }
             if (!fold.empty() || !rel.empty())
                 reliable_update(fold, rel);
+            // restart single-shift systems whose iterated residual was far below the true one
+            std::vector<int> restart;
+            for (int sys : rel) {
+                const CgmState<S> &c = state_h_[sys];
+                if (!c.stop && c.maxTerm <= 1 && !((double)c.rr < 1e2 * rr_iter[sys]))
+                    restart.push_back(sys);
+            }
+            if (!restart.empty()) {
+                set_list(restart);
+                copy_r_to_p<S, VR>(state_.as<CgmState<S>>(), vec(r_), p_.as<VR>(), off_.as<size_t>(), list(),
+                                   list_n(), max_sys_, st_);
+                cgm_restart_list<S>(state_.as<CgmState<S>>(), list(), list_n(), max_sys_, st_);
+                restarts_ += (int)restart.size();
+                download_states();
+                if (dbg)
+                    fprintf(stderr, "[kq] restarted %zu single-shift systems\n", restart.size());
+            }
             if (on_drop)
                 for (const auto &d : drops)
                     on_drop(d.first, d.second);
@@ -309,6 +333,7 @@ public:
     const CgmState<S> &state(int sys) const { return state_h_[sys]; }
     int nsys() const { return nsys_; }
     int reliable_updates() const { return reliable_updates_; }
+    int restarts() const { return restarts_; }
     const Geom &geom() const { return g_; }
 
 private:
@@ -509,6 +534,7 @@ private:
     double delta_ = 0.1, pause2_ = 0.0;
     DevBuf b_hi_, x_hi_, tmp64_, ax64_;
     int reliable_updates_ = 0;
+    int restarts_ = 0;
     hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
