@@ -34,7 +34,8 @@ struct CgmLaunch {
     uint32_t Vh;
     uint32_t nblk;      // blocks of the BLAS site loops (= stride of part_b)
     uint32_t nblk_dsl;  // blocks of the hopping-term launch (= stride of part_a)
-    int nsys;           // system slots
+    int nsys;           // system slots (k_cgm_begin scans all of them)
+    int nlaunch;        // slots of the live list the per-system kernels cover (>= live systems)
     int *rlist;         // compact list of live systems (written by k_cgm_begin)
     int *nact;          // its length
     int *any;           // 1 while any system is live

@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(256) k_cgm_scalars1(CgmState<S> *st, const int
 
 template <typename S> void cgm_scalars1(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_cgm_scalars1<S>, dim3(L.nsys), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_a,
+    hipLaunchKernelGGL(k_cgm_scalars1<S>, dim3(L.nlaunch), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_a,
                        L.nblk_dsl);
     KQ_HIP_CHECK(hipGetLastError());
 }
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(256) k_cgm_scalars2(CgmState<S> *st, const int
 
 template <typename S> void cgm_scalars2(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_cgm_scalars2<S>, dim3(L.nsys), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_b,
+    hipLaunchKernelGGL(k_cgm_scalars2<S>, dim3(L.nlaunch), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_b,
                        L.nblk);
     KQ_HIP_CHECK(hipGetLastError());
 }
@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(kBlasNT) k_axpy_norm(const CgmState<S> *st, Ve
 template <typename S, typename VR>
 void cgm_axpy_norm(CgmState<S> *st, Vec<VR> r, Vec<VR> s, const CgmLaunch &L, hipStream_t stream)
 {
-    hipLaunchKernelGGL((k_axpy_norm<S, VR>), dim3(L.nblk, L.nsys), dim3(kBlasNT), 0, stream, st, r, s, L.rlist,
+    hipLaunchKernelGGL((k_axpy_norm<S, VR>), dim3(L.nblk, L.nlaunch), dim3(kBlasNT), 0, stream, st, r, s, L.rlist,
                        L.nact, L.part_b, L.nblk);
     KQ_HIP_CHECK(hipGetLastError());
 }
@@ -201,17 +201,23 @@ __global__ void __launch_bounds__(kBlasNT) k_xp_update(const CgmState<S> *st, XR
         const VR zk = (VR)c.Z[k], ak = (VR)c.a[k];
         V2 *pp = reinterpret_cast<V2 *>(p) + base;
         X2 *xx = reinterpret_cast<X2 *>(x) + base;
+        // streaming accesses except the store of the unshifted direction p_0,
+        // which the next hopping term gathers: the solutions and the shifted
+        // directions are not touched again before the next update
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
-            const V2 pv = pp[(size_t)cc * Vh];
-            X2 xv = xx[(size_t)cc * Vh];
+            const V2 pv = ld_nt(pp + (size_t)cc * Vh);
+            X2 xv = ld_nt(xx + (size_t)cc * Vh);
             xv.x -= bk * (XR)pv.x;
             xv.y -= bk * (XR)pv.y;
-            xx[(size_t)cc * Vh] = xv;
+            st_nt(xx + (size_t)cc * Vh, xv);
             V2 np;
             np.x = zk * rv[cc].x + ak * pv.x;
             np.y = zk * rv[cc].y + ak * pv.y;
-            pp[(size_t)cc * Vh] = np;
+            if (k == 0)
+                pp[(size_t)cc * Vh] = np;
+            else
+                st_nt(pp + (size_t)cc * Vh, np);
         }
     }
 }
@@ -219,7 +225,7 @@ __global__ void __launch_bounds__(kBlasNT) k_xp_update(const CgmState<S> *st, XR
 template <typename S, typename VR, typename XR>
 void cgm_xp_update(CgmState<S> *st, XR *x, VR *p, Vec<VR> r, const CgmLaunch &L, hipStream_t stream)
 {
-    hipLaunchKernelGGL((k_xp_update<S, VR, XR>), dim3(L.nblk, L.nsys), dim3(kBlasNT), 0, stream, st, x, p, r, L.off,
+    hipLaunchKernelGGL((k_xp_update<S, VR, XR>), dim3(L.nblk, L.nlaunch), dim3(kBlasNT), 0, stream, st, x, p, r, L.off,
                        L.rlist, L.nact);
     KQ_HIP_CHECK(hipGetLastError());
 }

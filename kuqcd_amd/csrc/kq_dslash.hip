@@ -10,6 +10,8 @@
 // device links.
 #include "kq_dslash.h"
 
+#include <type_traits>
+
 namespace kq {
 
 // gather the colour vector at site n of right-hand side j (base pointer src[j])
@@ -51,6 +53,16 @@ __device__ __forceinline__ void hop_bwd(const A &a, const LK &lk, uint32_t s, co
         gather(src[j], n, a.g.Vh, v);
         udmv_sub(U, v, acc[j]);
     }
+}
+
+// output store: streaming for the FP64 and FP32 link formats (measured 5% and 4%
+// faster at 32^4), plain for the 16-bit one (3% slower with streaming stores)
+template <class LF, class V2> __device__ __forceinline__ void put(V2 *p, V2 v)
+{
+    if constexpr (std::is_same<LF, LinkH>::value)
+        *p = v;
+    else
+        st_nt(p, v);
 }
 
 template <class LF, class LL, typename SR, int NR, int EPI, int NT>
@@ -117,7 +129,7 @@ __global__ void __launch_bounds__(NT) k_dslash(const DslashArgs<LF, LL, SR> a)
                 // store is unconditional (same value to the same address)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    dst[j][(size_t)c * Vh + s] = mk<V2>((SR)acc[j][2 * c], (SR)acc[j][2 * c + 1]);
+                    put<LF>(&dst[j][(size_t)c * Vh + s], mk<V2>((SR)acc[j][2 * c], (SR)acc[j][2 * c + 1]));
             }
         }
     } else {
@@ -132,7 +144,7 @@ __global__ void __launch_bounds__(NT) k_dslash(const DslashArgs<LF, LL, SR> a)
                     const V2 p = aux[j][(size_t)c * Vh + s];
                     const SR ox = (SR)((R)sg * (R)p.x - acc[j][2 * c]);
                     const SR oy = (SR)((R)sg * (R)p.y - acc[j][2 * c + 1]);
-                    dst[j][(size_t)c * Vh + s] = mk<V2>(ox, oy);
+                    put<LF>(&dst[j][(size_t)c * Vh + s], mk<V2>(ox, oy));
                     d += (double)p.x * (double)ox + (double)p.y * (double)oy;
                 }
             }

@@ -30,6 +30,30 @@ __device__ inline kq_v4u ld_nt(const kq_v4u *p) { return __builtin_nontemporal_l
 __device__ inline kq_v2u ld_nt(const kq_v2u *p) { return __builtin_nontemporal_load(p); }
 __device__ inline float ld_nt(const float *p) { return __builtin_nontemporal_load(p); }
 
+// Streaming store of one complex number: the hopping term's output is not read
+// again by the same launch, so its lines should not evict the neighbour colour
+// vectors that other sites still gather from L2.
+__device__ inline void st_nt(double2 *p, double2 v)
+{
+    const kq_v2d t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<kq_v2d *>(p));
+}
+__device__ inline void st_nt(float2 *p, float2 v)
+{
+    const kq_v2f t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<kq_v2f *>(p));
+}
+__device__ inline double2 ld_nt(const double2 *p)
+{
+    const kq_v2d v = __builtin_nontemporal_load(reinterpret_cast<const kq_v2d *>(p));
+    return mk<double2>(v.x, v.y);
+}
+__device__ inline float2 ld_nt(const float2 *p)
+{
+    const kq_v2f v = __builtin_nontemporal_load(reinterpret_cast<const kq_v2f *>(p));
+    return mk<float2>(v.x, v.y);
+}
+
 // two 16-bit fixed-point reals (low half: real part) -> floats in units of the
 // stored scale: q / 32767 for |q| <= 32767
 __device__ inline float2 q2f(unsigned int u)

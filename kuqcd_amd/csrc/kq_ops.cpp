@@ -203,9 +203,11 @@ std::pair<int, int> tuned_launch(int slot, int mode, int nrhs)
     auto it = c.tune.find(key);
     if (it != c.tune.end())
         return it->second;
+    // blockings up to the smallest power of two >= nrhs (the kernel pads a
+    // partial block with the last live right-hand side, writing nothing for it)
     std::vector<int> nrs = {1};
     for (int v : {2, 4, 8})
-        if (v <= nrhs && !(mode == KQ_MODE_FP64 && v > 2))
+        if (v < 2 * nrhs && !(mode == KQ_MODE_FP64 && v > 2))
             nrs.push_back(v);
     double best = 1e30;
     std::pair<int, int> pick{1, 256};

@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <vector>
 
 #include "kq_cgm_kernels.h"
@@ -171,8 +172,8 @@ public:
     ~CgmRunner()
     {
         for (auto &g : graphs_)
-            if (g)
-                (void)hipGraphExecDestroy(g);
+            if (g.second)
+                (void)hipGraphExecDestroy(g.second);
     }
     CgmRunner(const CgmRunner &) = delete;
     CgmRunner &operator=(const CgmRunner &) = delete;
@@ -252,7 +253,7 @@ public:
             int nlive = 0;
             for (int sys = 0; sys < nsys_; ++sys)
                 nlive += (!state_h_[sys].stop && !state_h_[sys].pause) ? 1 : 0;
-            hipGraphExec_t gx = graph_for(tune_.pick_nr(nlive));
+            hipGraphExec_t gx = graph_for(nlive);
             for (int c = 0; c < tune_.chunk; ++c)
                 KQ_HIP_CHECK(hipGraphLaunch(gx, st_));
             download_states();
@@ -345,13 +346,14 @@ private:
     int *list() const { return ints_.as<int>() + max_sys_ + 2; }
     int *list_n() const { return ints_.as<int>() + 2 * max_sys_ + 2; }
 
-    CgmLaunch launch() const
+    CgmLaunch launch(int nlaunch) const
     {
         CgmLaunch L;
         L.Vh = g_.Vh;
         L.nblk = nblk_;
         L.nblk_dsl = nblk_dsl_;
         L.nsys = max_sys_;
+        L.nlaunch = std::min(std::max(nlaunch, 1), max_sys_);
         L.rlist = rlist();
         L.nact = nact();
         L.any = any();
@@ -465,10 +467,11 @@ private:
     }
 
     // One iteration on the runner's stream (what the graphs hold), with the
-    // hopping term blocked nr right-hand sides per thread.
-    void iteration(int nr)
+    // hopping term blocked nr right-hand sides per thread and every per-system
+    // grid sized for nr * groups live systems.
+    void iteration(int nr, int groups)
     {
-        const CgmLaunch L = launch();
+        const CgmLaunch L = launch(nr * groups);
         CgmState<S> *st = state_.as<CgmState<S>>();
         cgm_begin<S>(st, L, st_);
         DslashArgs<LF, LL, VR> d{};
@@ -478,7 +481,7 @@ private:
         d.run = any();
         d.rlist = rlist();
         d.nact = nact();
-        d.nrhs = max_sys_;
+        d.nrhs = L.nlaunch;
         d.dst_parity = 1 - parity_;
         d.src = vec_p0();
         d.dst = vec(tmp_);
@@ -496,18 +499,23 @@ private:
         cgm_xp_update<S, VR, VR>(st, x_.as<VR>(), p_.as<VR>(), vec(r_), L, st_);
     }
 
-    hipGraphExec_t graph_for(int nr)
+    // graph of one iteration for nlive live systems: right-hand-side blocking nr
+    // from the tuning table and ceil(nlive / nr) groups (no grid slots for
+    // systems that cannot be live during the chunk)
+    hipGraphExec_t graph_for(int nlive)
     {
-        const int i = nr >= 8 ? 3 : nr >= 4 ? 2 : nr >= 2 ? 1 : 0;
-        if (!graphs_[i]) {
+        const int nr = tune_.pick_nr(nlive);
+        const int groups = std::max(1, (nlive + nr - 1) / nr);
+        hipGraphExec_t &gx = graphs_[{nr, groups}];
+        if (!gx) {
             hipGraph_t graph;
             KQ_HIP_CHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
-            iteration(1 << i);
+            iteration(nr, groups);
             KQ_HIP_CHECK(hipStreamEndCapture(st_, &graph));
-            KQ_HIP_CHECK(hipGraphInstantiate(&graphs_[i], graph, nullptr, nullptr, 0));
+            KQ_HIP_CHECK(hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0));
             KQ_HIP_CHECK(hipGraphDestroy(graph));
         }
-        return graphs_[i];
+        return gx;
     }
 
     Geom g_;
@@ -535,7 +543,7 @@ private:
     DevBuf b_hi_, x_hi_, tmp64_, ax64_;
     int reliable_updates_ = 0;
     int restarts_ = 0;
-    hipGraphExec_t graphs_[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs_;
 };
 
 }  // namespace kq
