@@ -7,11 +7,13 @@ phases folded in, Gaussian even-site source, mass m = 0.01, 12 shifts
 log-spaced in [1e-4, 1] (absolute shifts m^2 + s_k), FP64 true relative
 residual <= 1e-10 for every shift.
 
-One step = one complete mixed-precision multishift solve (FP32 links and
-vectors, FP64 reliable updates and recurrences, per-shift FP64 verification and
+One step = one complete mixed-precision multishift solve (by default FP32 fat
+links, 16-bit fixed-point long links and FP32 vectors in the iteration; FP64
+reliable updates, recurrence scalars, per-shift FP64 verification and
 refinement) with the source and the solutions resident in HBM. The line also
-carries the plain FP64 GPU multishift CG time on the same problem, the FP64
-hopping-term roofline, and the oracle CGM timed on the host cores.
+carries the plain FP64 GPU multishift CG time on the same problem, the
+all-FP32-links mixed time, the FP64 hopping-term roofline, and the oracle CGM
+timed on the host cores.
 """
 from __future__ import annotations
 
@@ -30,6 +32,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "12-shift CGM time to rel. residual 1e-10, 32^4: mixed vs FP64; dslash %HBM peak"
 HBM_PEAK_GBS = 8000.0
+# arithmetic of each mixed mode: link storage + vectors + reliable updates
+MODE_DTYPE = {2: "fp32+fp64", 3: "int16-fixed+fp32+fp64", 4: "fp32+int16-fixed(long links)+fp64"}
+MODE_NAME = {2: "mixed: FP32 links", 3: "mixed: 16-bit links", 4: "mixed: FP32 fat + 16-bit long links"}
 
 
 def log(*a):
@@ -54,7 +59,8 @@ def main() -> int:
     ap.add_argument("--nshift", type=int, default=12)
     ap.add_argument("--mass", type=float, default=0.01)
     ap.add_argument("--rel-tol", type=float, default=1e-10)
-    ap.add_argument("--mode", type=int, default=2, help="2 = mixed FP32, 3 = mixed FP16")
+    ap.add_argument("--mode", type=int, default=4,
+                    help="mixed mode: 2 = FP32 links, 3 = 16-bit links, 4 = FP32 fat + 16-bit long links")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -64,7 +70,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build has no multi-GPU path "
-              "(the lattice would be split by 4-D domain decomposition with halo exchange; see DESIGN.md).",
+              "(the lattice would be split by 4-D domain decomposition with halo exchange).",
               file=sys.stderr)
         return 2
 
@@ -124,27 +130,49 @@ def main() -> int:
     for w in range(args.warmup):
         rep = solve(args.mode)
         log(f"warmup {w}: {rep.seconds:.3f}s iter {rep.iter}")
-    L.kq_end()
+    assert L.kq_synchronize() == 0
     t0 = time.perf_counter()
     reps = []
     for k in range(args.steps):
         reps.append(solve(args.mode))
         log(f"step {k}: {reps[-1].seconds:.3f}s iter {reps[-1].iter} refine {list(reps[-1].refine_iter[:args.nshift])}")
-    L.kq_end()
+    assert L.kq_synchronize() == 0
     t1 = time.perf_counter()
     sec_mixed = (t1 - t0) / args.steps
     rep = reps[-1]
     res_mixed = [rep.true_rel_res[k] for k in range(args.nshift)]
+
+    def download():
+        out = []
+        for p in x_dev:
+            h = np.empty_like(b)
+            assert L.kq_memcpy(h.ctypes.data, p, nbytes) == 0
+            out.append(h)
+        return out
+
+    x_mixed = download()
+
+    # the all-FP32-links mixed mode, for comparison
+    sec_fp32 = None
+    if args.mode != 2:
+        solve(2)  # warm (autotune / derived formats)
+        t = time.perf_counter()
+        rep32 = solve(2)
+        assert L.kq_synchronize() == 0
+        sec_fp32 = time.perf_counter() - t
+        log(f"mixed with FP32 links: {sec_fp32:.3f}s iter {rep32.iter}")
 
     # plain FP64 GPU multishift CG on the same problem
     solve(1)  # warm (autotune / derived formats)
     t2 = time.perf_counter()
     for _ in range(args.fp64_steps):
         rep64 = solve(1)
-    L.kq_end()
+    assert L.kq_synchronize() == 0
     t3 = time.perf_counter()
     sec_fp64 = (t3 - t2) / args.fp64_steps
     res_fp64 = [rep64.true_rel_res[k] for k in range(args.nshift)]
+    # per-shift solution difference, mixed vs the FP64 solve
+    diff = [float(np.linalg.norm(xm - x6) / np.linalg.norm(x6)) for xm, x6 in zip(x_mixed, download())]
     log(f"fp64: {sec_fp64:.3f}s iter {rep64.iter}; mixed {sec_mixed:.3f}s")
 
     # hopping-term roofline (HIP events on the library stream)
@@ -156,7 +184,7 @@ def main() -> int:
         log(f"dslash {name}: {r.ms * 1e3:.1f} us {r.gbs:.0f} GB/s {100 * r.hbm_frac:.1f}% nt {r.nt}")
     st, cp = A.bench_copy(1 << 30, 20)
     r64 = rl["fp64"]
-    traffic = pmc_traffic(ROOT / "profiles" / "dslash_fp64_32x32x32x32_pmc.txt")
+    traffic = pmc_traffic(ROOT / "profiles" / "dslash_32x32x32x32_pmc.txt")
 
     # CPU baseline: the oracle CGM (same algorithm) on the host cores, bounded sample
     cpu = None
@@ -193,23 +221,26 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64" if args.mode == 2 else "fp16+fp32+fp64",
+        "dtype": MODE_DTYPE[args.mode],
         "data": "synthetic (seeded weak-field SU(3) links, Gaussian source)",
         "config": {"workload": f"HISQ even-odd multishift CG, {'x'.join(map(str, lat))}, {args.nshift} shifts "
                                f"log-spaced 1e-4..1, m={args.mass}, FP64 true rel. residual {args.rel_tol:g} per shift",
                    "lattice": list(lat), "nshift": args.nshift, "mass": args.mass,
-                   "mode": "mixed-fp32" if args.mode == 2 else "mixed-fp16", "parallelism": "single GPU"},
+                   "mode": MODE_NAME[args.mode], "parallelism": "single GPU"},
         "fp64_seconds": round(sec_fp64, 4),
         "speedup_vs_fp64": round(sec_fp64 / sec_mixed, 3),
+        "mixed_fp32_links_seconds": round(sec_fp32, 4) if sec_fp32 else None,
         "iterations": {"mixed_multishift": rep.iter, "mixed_reliable_updates": rep.reliable_updates,
                        "mixed_refine": [rep.refine_iter[k] for k in range(args.nshift)], "fp64": rep64.iter},
         "max_true_rel_res": {"mixed": max(res_mixed), "fp64": max(res_fp64)},
+        "rel_diff_mixed_vs_fp64": [float(f"{d:.2e}") for d in diff],
         "roofline": {"bound": "hbm", "achieved": round(r64.gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r64.gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_dslash<LinkD,LinkD,double> (FP64 hopping term, 2400 B/site)",
                      "us_per_launch": round(r64.ms * 1e3, 2), "gflops": round(r64.gflops, 1)},
         "roofline_fp32": {"achieved": round(rl["fp32"].gbs, 1), "frac": round(rl["fp32"].hbm_frac, 4),
-                          "us_per_launch": round(rl["fp32"].ms * 1e3, 2), "bytes_per_site": 1200},
+                          "us_per_launch": round(rl["fp32"].ms * 1e3, 2),
+                          "bytes_per_site": round(rl["fp32"].bytes / Vh, 1)},
         "roofline_fp16": {"achieved": round(rl["fp16"].gbs, 1), "frac": round(rl["fp16"].hbm_frac, 4),
                           "us_per_launch": round(rl["fp16"].ms * 1e3, 2),
                           "bytes_per_site": round(rl["fp16"].bytes / Vh, 1)},

@@ -42,8 +42,10 @@ typedef enum {
     KQ_MODE_REFERENCE = 0,  /* the reference's method in the loaded gauge precision */
     KQ_MODE_FP64 = 1,       /* plain FP64 multishift CG (double gauge) */
     KQ_MODE_MIXED_FP32 = 2, /* FP32 links/vectors, FP64 reliable updates + per-shift refinement */
-    KQ_MODE_MIXED_FP16 = 3  /* 16-bit links: fixed-point int16 entries with a per-link FP32 scale
+    KQ_MODE_MIXED_FP16 = 3, /* 16-bit links: fixed-point int16 entries with a per-link FP32 scale
                                (12-real long links), FP32 vectors */
+    KQ_MODE_MIXED_FP32_L16 = 4 /* FP32 fat links, 16-bit long links (the Naik term is 1/24 of the
+                                  fat term, so its rounding costs the solve little), FP32 vectors */
 } KqMode;
 
 typedef struct {
@@ -131,6 +133,8 @@ KQ_API void *kq_device_alloc(size_t bytes);
 KQ_API void kq_device_free(void *p);
 /* copy `bytes` between any two host/device pointers; returns 0 on success */
 KQ_API int kq_memcpy(void *dst, const void *src, size_t bytes);
+/* wait until all work the library has queued on its stream is done; returns 0 on success */
+KQ_API int kq_synchronize(void);
 
 /* ---- learned preconditioner (no reference counterpart)
  * A two-layer parallel-transport convolution M built from the loaded fat and

@@ -33,7 +33,7 @@ KUQCD_INVALID_RECON = 0
 KUQCD_WILSON_DIRAC, KUQCD_WILSONPC_DIRAC, KUQCD_HISQ_DIRAC, KUQCD_HISQPC_DIRAC, KUQCD_INVALID_DIRAC = range(5)
 KUQCD_CG, KUQCD_CGM, KUQCD_BICGSTAB, KUQCD_BICGSTABM, KUQCD_INVALID_INVERTER = range(5)
 
-KQ_MODE_REFERENCE, KQ_MODE_FP64, KQ_MODE_MIXED_FP32, KQ_MODE_MIXED_FP16 = 0, 1, 2, 3
+KQ_MODE_REFERENCE, KQ_MODE_FP64, KQ_MODE_MIXED_FP32, KQ_MODE_MIXED_FP16, KQ_MODE_MIXED_FP32_L16 = 0, 1, 2, 3, 4
 KQ_MAX_SHIFT = 32
 KQ_GEN_HAAR, KQ_GEN_WEAK = 0, 1
 
@@ -199,6 +199,7 @@ _PROTOS = {
     "kq_device_alloc": (_VP, [C.c_size_t]),
     "kq_device_free": (None, [_VP]),
     "kq_memcpy": (C.c_int, [_VP, _VP, C.c_size_t]),
+    "kq_synchronize": (C.c_int, []),
     "kq_ptnet_init_weights": (None, [C.POINTER(KqPtWeights), C.c_uint64, C.c_double]),
     "kq_ptnet_set_weights": (C.c_int, [C.POINTER(KqPtWeights)]),
     "kq_ptnet_get_weights": (C.c_int, [C.POINTER(KqPtWeights)]),

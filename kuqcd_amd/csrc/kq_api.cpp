@@ -38,6 +38,7 @@ void ensure_device()
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
         throw std::runtime_error("no HIP device is visible");
     KQ_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    DevPool::get().set_stream(c.stream);
     c.device_ready = true;
 }
 
@@ -416,6 +417,8 @@ static void free_all_gauges()
         c.der[i] = DerivedFormats();
     }
     c.tune.clear();
+    c.tune_group.clear();
+    DevPool::get().trim();
 }
 
 }  // namespace kq
@@ -465,6 +468,7 @@ void kq_end(void)
         if (c.device_ready)
             (void)hipStreamSynchronize(c.stream);
         learned_release();
+        DevPool::get().trim();
     } catch (...) {
     }
 }
@@ -551,6 +555,17 @@ void kq_device_free(void *p)
 {
     if (p)
         (void)hipFree(p);
+}
+int kq_synchronize(void)
+{
+    try {
+        ensure_device();
+        KQ_HIP_CHECK(hipStreamSynchronize(ctx().stream));
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
 }
 int kq_memcpy(void *dst, const void *src, size_t bytes)
 {

@@ -3,6 +3,7 @@
 // (the reference API is likewise not re-entrant, kuqcd.cpp:17-38).
 #pragma once
 
+#include <array>
 #include <map>
 #include <string>
 #include <tuple>
@@ -52,6 +53,9 @@ struct Context {
     std::string last_error;
     // autotuner cache: (mode, nrhs, Vh) -> (nr, nt)
     std::map<std::tuple<int, int, uint32_t>, std::pair<int, int>> tune;
+    // (mode, Vh) -> milliseconds of one group of NR = 1, 2, 4, 8 right-hand sides at block
+    // size 64, 128, 256 (index 3 * log2(NR) + log2(nt / 64)); < 0 where not measured
+    std::map<std::pair<int, uint32_t>, std::array<double, 12>> tune_group;
 };
 
 Context &ctx();
@@ -68,11 +72,25 @@ DerivedFormats &derived(int slot, bool want_f32, bool want_f16);
 
 // fn(fat, lng) with the device link formats of a mixed mode of a double slot:
 // KQ_MODE_MIXED_FP32 -> LinkF fat, LinkF12 long links if they are b * SU(3) else LinkF;
-// KQ_MODE_MIXED_FP16 -> LinkH fat, LinkH12 long links if they are b * SU(3) else LinkH.
+// KQ_MODE_MIXED_FP16 -> LinkH fat, LinkH12 long links if they are b * SU(3) else LinkH;
+// KQ_MODE_MIXED_FP32_L16 -> LinkF fat, long links as in KQ_MODE_MIXED_FP16.
 // Returns whether the long links use 12 reals.
+inline bool is_mixed_mode(int mode)
+{
+    return mode == KQ_MODE_MIXED_FP32 || mode == KQ_MODE_MIXED_FP16 || mode == KQ_MODE_MIXED_FP32_L16;
+}
+
 template <class Fn> bool with_mixed_links(int slot, int mode, Fn &&fn)
 {
     const uint32_t Vh = slot_geom(slot).Vh;
+    if (mode == KQ_MODE_MIXED_FP32_L16) {
+        DerivedFormats &d = derived(slot, true, true);
+        if (d.long12)
+            fn(as_linkF(d.fat_f, Vh), as_linkH12(d.lng_h, Vh));
+        else
+            fn(as_linkF(d.fat_f, Vh), as_linkH(d.lng_h, Vh));
+        return d.long12;
+    }
     if (mode == KQ_MODE_MIXED_FP16) {
         DerivedFormats &d = derived(slot, false, true);
         if (d.long12)
@@ -82,7 +100,7 @@ template <class Fn> bool with_mixed_links(int slot, int mode, Fn &&fn)
         return d.long12;
     }
     if (mode != KQ_MODE_MIXED_FP32)
-        throw std::invalid_argument("link format must be KQ_MODE_MIXED_FP32 or KQ_MODE_MIXED_FP16");
+        throw std::invalid_argument("link format must be a mixed mode (KQ_MODE_MIXED_*)");
     DerivedFormats &d = derived(slot, true, false);
     if (d.f32_long12)
         fn(as_linkF(d.fat_f, Vh), as_linkF12(d.lng_f12, Vh));

@@ -214,6 +214,8 @@ template void launch_dslash<LinkD, LinkD, double>(DslashArgs<LinkD, LinkD, doubl
 template void launch_dslash<LinkF, LinkF, float>(DslashArgs<LinkF, LinkF, float>, int, int, int, hipStream_t);
 template void launch_dslash<LinkF, LinkF12, float>(DslashArgs<LinkF, LinkF12, float>, int, int, int, hipStream_t);
 template void launch_dslash<LinkH, LinkH, float>(DslashArgs<LinkH, LinkH, float>, int, int, int, hipStream_t);
+template void launch_dslash<LinkF, LinkH, float>(DslashArgs<LinkF, LinkH, float>, int, int, int, hipStream_t);
+template void launch_dslash<LinkF, LinkH12, float>(DslashArgs<LinkF, LinkH12, float>, int, int, int, hipStream_t);
 template void launch_dslash<LinkH, LinkH12, float>(DslashArgs<LinkH, LinkH12, float>, int, int, int, hipStream_t);
 
 }  // namespace kq

@@ -108,7 +108,7 @@ static KuQCDStatus apply(int nrhs, const void *src, void *dst, int parity, int m
         set_error("mixed modes need a double-precision gauge field");
         return KUQCD_ERR_UNSUPPORTED;
     }
-    if (mode == KQ_MODE_MIXED_FP32 || mode == KQ_MODE_MIXED_FP16) {
+    if (is_mixed_mode(mode)) {
         with_mixed_links(slot, mode, [&](auto fat, auto lng) {
             apply_t<decltype(fat), decltype(lng), float, double>(fat, lng, g, nrhs, src, dst, parity, nr, nt, mdaggm,
                                                                  sigma);
@@ -159,8 +159,8 @@ static double time_dslash(LF fat, LL lng, const Geom &g, int nrhs, int nr, int n
 static double site_bytes(int mode, bool dbl_gauge, bool long12, int nrhs)
 {
     double bl_f, bl_l, bv;
-    if (mode == KQ_MODE_MIXED_FP16) {
-        bl_f = 36 + 4;
+    if (mode == KQ_MODE_MIXED_FP16 || mode == KQ_MODE_MIXED_FP32_L16) {
+        bl_f = mode == KQ_MODE_MIXED_FP16 ? 36 + 4 : 72;
         bl_l = long12 ? 24 + 4 : 36 + 4;
         bv = 24;
     } else if (mode == KQ_MODE_MIXED_FP32 || !dbl_gauge) {
@@ -203,23 +203,33 @@ std::pair<int, int> tuned_launch(int slot, int mode, int nrhs)
     auto it = c.tune.find(key);
     if (it != c.tune.end())
         return it->second;
-    // blockings up to the smallest power of two >= nrhs (the kernel pads a
-    // partial block with the last live right-hand side, writing nothing for it)
-    std::vector<int> nrs = {1};
-    for (int v : {2, 4, 8})
-        if (v < 2 * nrhs && !(mode == KQ_MODE_FP64 && v > 2))
-            nrs.push_back(v);
+    // Each blocking NR is timed once, on a single group of NR right-hand sides, and a launch
+    // for nrhs systems is priced as ceil(nrhs / NR) such groups (the groups are independent
+    // workgroups of the same kernel). Blockings go up to the smallest power of two >= nrhs:
+    // the kernel pads a partial block with the last live right-hand side and writes nothing
+    // for it. FP64 stops at NR = 2 (register pressure).
+    static const int kNr[4] = {1, 2, 4, 8}, kNt[3] = {64, 128, 256};
+    auto &tg = c.tune_group[{mode, g.Vh}];
+    if (tg[0] == 0.0)
+        tg.fill(-1.0);
     double best = 1e30;
     std::pair<int, int> pick{1, 256};
     bool l12 = false;
-    for (int nr : nrs)
-        for (int nt : {64, 128, 256}) {
-            const double ms = time_mode(slot, mode, nrhs, nr, nt, 8, &l12);
-            if (ms < best) {
-                best = ms;
-                pick = {nr, nt};
+    for (int i = 0; i < 4; ++i) {
+        const int nr = kNr[i];
+        if (nr > 1 && (nr >= 2 * nrhs || (mode == KQ_MODE_FP64 && nr > 2)))
+            continue;
+        for (int j = 0; j < 3; ++j) {
+            double &ms = tg[3 * i + j];
+            if (ms < 0)
+                ms = time_mode(slot, mode, nr, nr, kNt[j], 8, &l12);
+            const double cost = ms * ((nrhs + nr - 1) / nr);
+            if (cost < best) {
+                best = cost;
+                pick = {nr, kNt[j]};
             }
         }
+    }
     c.tune[key] = pick;
     if (c.verbosity > 1)
         log_info("autotune mode %d nrhs %d: nr %d nt %d (%.3f ms)\n", mode, nrhs, pick.first, pick.second, best);

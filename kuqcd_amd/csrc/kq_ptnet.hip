@@ -600,5 +600,7 @@ KQ_PT_INST(LinkF, LinkF)
 KQ_PT_INST(LinkF, LinkF12)
 KQ_PT_INST(LinkH, LinkH12)
 KQ_PT_INST(LinkH, LinkH)
+KQ_PT_INST(LinkF, LinkH12)
+KQ_PT_INST(LinkF, LinkH)
 
 }  // namespace kq

@@ -466,7 +466,7 @@ KuQCDStatus solve_extended(int slot, int nrhs, void **x, void **b, KuQCDInvertPa
     const int mode = (o.mode == KQ_MODE_REFERENCE) ? KQ_MODE_FP64 : o.mode;
     if (mode == KQ_MODE_FP64) {
         run_fp64(P, xdev, rep, res2);
-    } else if (mode == KQ_MODE_MIXED_FP32 || mode == KQ_MODE_MIXED_FP16) {
+    } else if (is_mixed_mode(mode)) {
         with_mixed_links(slot, mode, [&](auto fat, auto lng) { run_mixed(P, fat, lng, xdev, rep, res2); });
     } else {
         set_error("unknown mode");

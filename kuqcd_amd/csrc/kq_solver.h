@@ -29,6 +29,68 @@
 
 namespace kq {
 
+// Device-memory cache: released buffers are kept by size and handed out again to
+// allocations of the same size, so repeated solves of one problem shape call
+// hipMalloc/hipFree only the first time. Before a cached block is reused the
+// library stream is synchronised (normally idle by then), so no work queued
+// under the block's previous owner can still touch it. trim() frees the cache
+// (gauge free, library end, and before retrying a failed hipMalloc).
+class DevPool {
+public:
+    static DevPool &get()
+    {
+        // never destroyed: buffers owned by other static objects are released at exit
+        static DevPool *pool = new DevPool;
+        return *pool;
+    }
+    void set_stream(hipStream_t s) { stream_ = s; }
+    void *take(size_t bytes)
+    {
+        auto it = free_.find(bytes);
+        if (it != free_.end()) {
+            void *p = it->second;
+            free_.erase(it);
+            if (stream_)
+                (void)hipStreamSynchronize(stream_);
+            else
+                (void)hipDeviceSynchronize();
+            return p;
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            trim();
+            if (hipMalloc(&p, bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
+        }
+        return p;
+    }
+    void give(void *p, size_t bytes) { free_.emplace(bytes, p); }
+    void trim()
+    {
+        if (free_.empty())
+            return;
+        if (stream_)
+            (void)hipStreamSynchronize(stream_);
+        for (auto &f : free_)
+            (void)hipFree(f.second);
+        free_.clear();
+    }
+    size_t cached_bytes() const
+    {
+        size_t n = 0;
+        for (const auto &f : free_)
+            n += f.first;
+        return n;
+    }
+
+private:
+    std::multimap<size_t, void *> free_;
+    hipStream_t stream_ = nullptr;
+};
+
 class DevBuf {
 public:
     DevBuf() = default;
@@ -56,18 +118,16 @@ public:
     {
         release();
         if (bytes) {
-            hipError_t e = hipMalloc(&p_, bytes);
-            if (e != hipSuccess) {
-                p_ = nullptr;
+            p_ = DevPool::get().take(bytes);
+            if (!p_)
                 throw std::bad_alloc();
-            }
         }
         n_ = bytes;
     }
     void release()
     {
         if (p_)
-            (void)hipFree(p_);
+            DevPool::get().give(p_, n_);
         p_ = nullptr;
         n_ = 0;
     }
@@ -253,9 +313,15 @@ public:
             int nlive = 0;
             for (int sys = 0; sys < nsys_; ++sys)
                 nlive += (!state_h_[sys].stop && !state_h_[sys].pause) ? 1 : 0;
-            hipGraphExec_t gx = graph_for(nlive);
-            for (int c = 0; c < tune_.chunk; ++c)
-                KQ_HIP_CHECK(hipGraphLaunch(gx, st_));
+            if (direct_launches()) {
+                const int nr = tune_.pick_nr(nlive);
+                for (int c = 0; c < tune_.chunk; ++c)
+                    iteration(nr, std::max(1, (nlive + nr - 1) / nr));
+            } else {
+                hipGraphExec_t gx = graph_for(nlive);
+                for (int c = 0; c < tune_.chunk; ++c)
+                    KQ_HIP_CHECK(hipGraphLaunch(gx, st_));
+            }
             download_states();
             std::vector<int> fold, rel;
             std::vector<double> rr_iter(nsys_, 0.0);
@@ -497,6 +563,17 @@ private:
         cgm_axpy_norm<S, VR>(st, vec(r_), vec(s_), L, st_);
         cgm_scalars2<S>(st, L, st_);
         cgm_xp_update<S, VR, VR>(st, x_.as<VR>(), p_.as<VR>(), vec(r_), L, st_);
+    }
+
+    // KQ_DIRECT_LAUNCH=1 launches the iteration's kernels one by one instead of
+    // replaying graphs (same kernels and grids; for tools that trace launches)
+    static bool direct_launches()
+    {
+        static const bool on = [] {
+            const char *e = getenv("KQ_DIRECT_LAUNCH");
+            return e != nullptr && e[0] == '1';
+        }();
+        return on;
     }
 
     // graph of one iteration for nlive live systems: right-hand-side blocking nr
