@@ -88,7 +88,7 @@ def main() -> int:
     log(f"inputs generated in {time.time() - t:.1f}s")
     A.load_gauge_pair(fat, lng, lat)
     if args.kernels_only:
-        for mode, name in ((1, "fp64"), (2, "fp32"), (3, "fp16")):
+        for mode, name in ((1, "fp64"), (2, "fp32"), (4, "fp32 fat + 16-bit long"), (3, "fp16")):
             st, r = A.bench_dslash(mode, 1, 0, 0, 20)
             assert st == 0, A.last_error()
             print(f"dslash {name}: {r.ms * 1e3:.1f} us, {r.bytes / Vh:.0f} B/site, {r.gbs:.0f} GB/s, "
@@ -177,7 +177,7 @@ def main() -> int:
 
     # hopping-term roofline (HIP events on the library stream)
     rl = {}
-    for mode, name in ((1, "fp64"), (2, "fp32"), (3, "fp16")):
+    for mode, name in ((1, "fp64"), (2, "fp32"), (4, "fp32_l16"), (3, "fp16")):
         st, r = A.bench_dslash(mode, 1, 0, 0, 50)
         assert st == 0, A.last_error()
         rl[name] = r
@@ -241,6 +241,9 @@ def main() -> int:
         "roofline_fp32": {"achieved": round(rl["fp32"].gbs, 1), "frac": round(rl["fp32"].hbm_frac, 4),
                           "us_per_launch": round(rl["fp32"].ms * 1e3, 2),
                           "bytes_per_site": round(rl["fp32"].bytes / Vh, 1)},
+        "roofline_fp32_l16": {"achieved": round(rl["fp32_l16"].gbs, 1), "frac": round(rl["fp32_l16"].hbm_frac, 4),
+                              "us_per_launch": round(rl["fp32_l16"].ms * 1e3, 2),
+                              "bytes_per_site": round(rl["fp32_l16"].bytes / Vh, 1)},
         "roofline_fp16": {"achieved": round(rl["fp16"].gbs, 1), "frac": round(rl["fp16"].hbm_frac, 4),
                           "us_per_launch": round(rl["fp16"].ms * 1e3, 2),
                           "bytes_per_site": round(rl["fp16"].bytes / Vh, 1)},

@@ -548,4 +548,24 @@ template void vec_dot_partial<double>(Vec<double>, Vec<double>, int, double *, u
 template void vec_dot_partial<float>(Vec<float>, Vec<float>, int, double *, uint32_t, hipStream_t, const int *,
                                      const int *);
 
+// ------------------------------------------------------------ small copies
+__global__ void __launch_bounds__(256) k_copy_words(const unsigned long long *src, unsigned long long *dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+void copy_words(const void *src, void *dst, size_t bytes, hipStream_t stream)
+{
+    if (bytes % 8 != 0)
+        throw std::invalid_argument("copy_words: size must be a multiple of 8 bytes");
+    const size_t n = bytes / 8;
+    if (n == 0)
+        return;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 64);
+    hipLaunchKernelGGL(k_copy_words, dim3(grid), dim3(256), 0, stream,
+                       static_cast<const unsigned long long *>(src), static_cast<unsigned long long *>(dst), n);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace kq

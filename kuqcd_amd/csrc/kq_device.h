@@ -75,6 +75,11 @@ struct Geom {
     uint32_t Vh;
     uint32_t stride[4];  // checkerboard index stride of y, z, t (stride[0] unused)
     FastDiv dx, dy, dz;  // divisions by Lx/2, Ly, Lz
+    // lattice tiles of the hopping term's block order (see tile_site): valid when
+    // Lz % 4 == 0 and Lt % 2 == 0; plane = (Lx/2)*Ly sites, zc = Lz/4, tc = Lt/2
+    int tile;
+    uint32_t plane, zc, tc;
+    FastDiv dplane, dzc;
 };
 
 __host__ inline Geom make_geom(const int L[4])
@@ -91,6 +96,12 @@ __host__ inline Geom make_geom(const int L[4])
     g.dx = FastDiv((uint32_t)g.lxh);
     g.dy = FastDiv((uint32_t)L[1]);
     g.dz = FastDiv((uint32_t)L[2]);
+    g.tile = (L[2] % 4 == 0 && L[3] % 2 == 0) ? 1 : 0;
+    g.plane = g.stride[2];
+    g.zc = (uint32_t)(L[2] / 4);
+    g.tc = (uint32_t)(L[3] / 2);
+    g.dplane = FastDiv(g.plane);
+    g.dzc = FastDiv(g.zc > 0 ? g.zc : 1);
     return g;
 }
 
@@ -178,6 +189,30 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb)
     if ((nb & 7u) != 0)
         return b;
     return (b & 7u) * (nb >> 3) + (b >> 3);
+}
+
+// Site of thread tid of block b (of nb blocks of NT threads) for the stencil kernels.
+// Workgroups are dealt to the 8 XCDs round robin (b % 8), each XCD with its own L2.
+// When Vh is a multiple of 8*NT and the lattice splits into 4 z-ranges x 2 t-ranges,
+// XCD x owns the tile z in [(x%4) Lz/4, ...), t in [(x/4) Lt/2, ...) and walks it
+// plane by plane ((x,y) fastest, then z, then t), so the +-1 and +-3 neighbours along
+// z and t that a site gathers were mostly just read by the same XCD. Otherwise each
+// XCD gets a contiguous range of blocks (xcd_remap). Either way a bijection onto the
+// sites (tile: v < Vh/8 = plane*zc*tc maps to distinct (p, zl, tl) per XCD).
+template <int NT> __device__ inline uint32_t tile_site(const Geom &g, uint32_t b, uint32_t nb, uint32_t tid, bool &valid)
+{
+    if (g.tile && (g.Vh % (8u * NT)) == 0) {
+        const uint32_t x = b & 7u, v = (b >> 3) * NT + tid;
+        const uint32_t q = g.dplane.div(v);
+        const uint32_t p = v - q * g.plane;
+        const uint32_t tl = g.dzc.div(q);
+        const uint32_t zl = q - tl * g.zc;
+        valid = true;
+        return p + g.plane * (((x & 3u) * g.zc + zl) + (uint32_t)g.L[2] * ((x >> 2) * g.tc + tl));
+    }
+    const uint32_t s0 = xcd_remap(b, nb) * NT + tid;
+    valid = s0 < g.Vh;
+    return valid ? s0 : 0;
 }
 
 }  // namespace kq

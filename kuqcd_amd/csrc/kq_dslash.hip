@@ -92,10 +92,8 @@ __global__ void __launch_bounds__(NT) k_dslash(const DslashArgs<LF, LL, SR> a)
         aux[j] = EPI == EPI_AXMY_DOT ? a.aux.p + a.aux.base(rr[j]) : nullptr;
     }
     const uint32_t Vh = a.g.Vh;
-    const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t s0 = blk * NT + threadIdx.x;
-    const bool valid = s0 < a.g.Vh;
-    const uint32_t s = valid ? s0 : 0;
+    bool valid;
+    const uint32_t s = tile_site<NT>(a.g, blockIdx.x, gridDim.x, threadIdx.x, valid);
     R acc[NR][6];
 #pragma unroll
     for (int j = 0; j < NR; ++j)

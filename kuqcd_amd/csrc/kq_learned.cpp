@@ -201,7 +201,7 @@ FcgOut fcg_solve(const Geom &g, int q, AF af, AL al, PF pf, PL pl, LinkD df, Lin
     H.max_iter = cfg.max_iter;
     H.rr_stop = cfg.tol2;
     auto upload = [&] {
-        KQ_HIP_CHECK(hipMemcpyAsync(S, &H, sizeof H, hipMemcpyHostToDevice, st));
+        copy_words(sh.dev(), S, sizeof H, st);
     };
     upload();
     KQ_HIP_CHECK(hipMemsetAsync(vx.as<void>(), 0, fb, st));
@@ -288,7 +288,7 @@ FcgOut fcg_solve(const Geom &g, int q, AF af, AL al, PF pf, PL pl, LinkD df, Lin
         hipGraphExec_t gx = graph_for(net);
         do {
             KQ_HIP_CHECK(hipGraphLaunch(gx, st));
-            KQ_HIP_CHECK(hipMemcpyAsync(&H, S, sizeof H, hipMemcpyDeviceToHost, st));
+            copy_words(S, sh.dev(), sizeof H, st);
             KQ_HIP_CHECK(hipStreamSynchronize(st));
         } while (!H.stop);
         if (net)

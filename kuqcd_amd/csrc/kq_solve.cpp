@@ -260,7 +260,9 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
         const double tol = 0.25 * P.eps[r] * P.eps[r] * P.b2[r];
         forked[(size_t)r * ns + k] = run.add_system(1, &sh, &tol, P.p->max_iter, run.b_hi(sys), run.x_hi(sys, k));
     };
+    const auto t_setup = clk::now();
     run.run(on_drop);
+    const auto t_ran = clk::now();
     for (int r = 0; r < nrhs; ++r) {
         rep[r].iter = run.state(sys_of[r]).iter;
         rep[r].reliable_updates = run.reliable_updates();
@@ -324,10 +326,13 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
     for (int r = 0; r < nrhs; ++r)
         for (int k = 0; k < ns; ++k)
             rep[r].initial_rel_res[k] = std::sqrt(res2[(size_t)r * ns + k] / P.b2[r]);
-    if (debug_on())
+    if (debug_on()) {
+        fprintf(stderr, "[kq] mixed solve: setup %.2f ms, iterations %.2f ms, copy-out %.2f ms, verification %.2f ms\n",
+                1e3 * secs(t0, t_setup), 1e3 * secs(t_setup, t_ran), 1e3 * secs(t_ran, t1), 1e3 * t_verify);
         for (int k = 0; k < ns; ++k)
             fprintf(stderr, "[kq] shift %d true rel res after the main phase %.3e (fork iters %d)\n", k,
                     rep[0].initial_rel_res[k], rep[0].refine_iter[k]);
+    }
     auto t2 = clk::now();
     if (P.o->refine) {
         for (int round = 0; round < 3; ++round) {

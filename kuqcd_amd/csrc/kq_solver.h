@@ -155,6 +155,12 @@ public:
                 p_ = nullptr;
                 throw std::bad_alloc();
             }
+            // pinned host memory is mapped into the device address space
+            if (hipHostGetDevicePointer(reinterpret_cast<void **>(&d_), p_, 0) != hipSuccess) {
+                (void)hipHostFree(p_);
+                p_ = nullptr;
+                throw std::bad_alloc();
+            }
         }
         n_ = n;
     }
@@ -163,13 +169,16 @@ public:
         if (p_)
             (void)hipHostFree(p_);
         p_ = nullptr;
+        d_ = nullptr;
         n_ = 0;
     }
     T *get() const { return p_; }
+    T *dev() const { return d_; }  // the same memory as seen by kernels
     T &operator[](size_t i) const { return p_[i]; }
 
 private:
     T *p_ = nullptr;
+    T *d_ = nullptr;
     size_t n_ = 0;
 };
 
@@ -308,8 +317,14 @@ public:
     {
         start_pending();
         static const bool dbg = getenv("KQ_DEBUG") != nullptr;
+        using clk = std::chrono::steady_clock;
+        double t_launch = 0, t_wait = 0, t_rel = 0;
+        int nchunks = 0;
+        const auto t_run0 = clk::now();
         for (;;) {
             // one graph per right-hand-side blocking, chosen from the live count
+            const auto ta = clk::now();
+            ++nchunks;
             int nlive = 0;
             for (int sys = 0; sys < nsys_; ++sys)
                 nlive += (!state_h_[sys].stop && !state_h_[sys].pause) ? 1 : 0;
@@ -322,7 +337,11 @@ public:
                 for (int c = 0; c < tune_.chunk; ++c)
                     KQ_HIP_CHECK(hipGraphLaunch(gx, st_));
             }
+            const auto tb = clk::now();
             download_states();
+            const auto tc = clk::now();
+            t_launch += std::chrono::duration<double>(tb - ta).count();
+            t_wait += std::chrono::duration<double>(tc - tb).count();
             std::vector<int> fold, rel;
             std::vector<double> rr_iter(nsys_, 0.0);
             std::vector<std::pair<int, int>> drops;
@@ -363,8 +382,11 @@ public:
                     rel.push_back(sys);
                 }
             }
-            if (!fold.empty() || !rel.empty())
+            if (!fold.empty() || !rel.empty()) {
+                const auto tr = clk::now();
                 reliable_update(fold, rel);
+                t_rel += std::chrono::duration<double>(clk::now() - tr).count();
+            }
             // restart single-shift systems whose iterated residual was far below the true one
             std::vector<int> restart;
             for (int sys : rel) {
@@ -392,6 +414,11 @@ public:
             if (!live)
                 break;
         }
+        if (dbg)
+            fprintf(stderr, "[kq] run: %.2f ms in %d chunks: launch %.2f ms, wait for chunk %.2f ms, reliable "
+                            "updates %.2f ms\n",
+                    1e3 * std::chrono::duration<double>(clk::now() - t_run0).count(), nchunks, 1e3 * t_launch,
+                    1e3 * t_wait, 1e3 * t_rel);
     }
 
     double *x_hi(int sys, int k) const { return x_hi_.as<double>() + 2 * (off_h_[sys] + (size_t)k * n3_); }
@@ -429,17 +456,15 @@ private:
         return L;
     }
 
+    // the states move with a copy kernel through the mapped pinned buffer (a DMA
+    // transfer would add a copy-engine round trip after every chunk)
+    static_assert(sizeof(CgmState<S>) % 8 == 0, "CgmState is copied in 8-byte words");
     void download_states()
     {
-        KQ_HIP_CHECK(hipMemcpyAsync(state_h_.get(), state_.as<void>(), sizeof(CgmState<S>) * max_sys_,
-                                    hipMemcpyDeviceToHost, st_));
+        copy_words(state_.as<void>(), state_h_.dev(), sizeof(CgmState<S>) * max_sys_, st_);
         KQ_HIP_CHECK(hipStreamSynchronize(st_));
     }
-    void upload_states()
-    {
-        KQ_HIP_CHECK(hipMemcpyAsync(state_.as<void>(), state_h_.get(), sizeof(CgmState<S>) * max_sys_,
-                                    hipMemcpyHostToDevice, st_));
-    }
+    void upload_states() { copy_words(state_h_.dev(), state_.as<void>(), sizeof(CgmState<S>) * max_sys_, st_); }
     // upload a list of systems for the list-driven kernels (blocking)
     void set_list(const std::vector<int> &l)
     {
