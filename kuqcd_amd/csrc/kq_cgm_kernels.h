@@ -78,6 +78,9 @@ void cgm_restart_list(CgmState<S> *st, const int *list, const int *n, int nsys, 
 // copy `bytes` (a multiple of 8) with a kernel; either side may be mapped pinned host
 // memory, which avoids a copy-engine round trip for the small state transfers
 void copy_words(const void *src, void *dst, size_t bytes, hipStream_t stream);
+
+// list[0..n) = src[0..n), *count = n (src may be mapped pinned host memory)
+void set_int_list(const int *src, int *list, int *count, int n, hipStream_t stream);
 // x_hi[k] += x_lo[k]; x_lo[k] = 0 for k < st[sys].maxTermFold
 template <typename S, typename VR>
 void rel_fold(CgmState<S> *st, double *x_hi, VR *x_lo, const size_t *off, uint32_t Vh, const int *list, const int *n,

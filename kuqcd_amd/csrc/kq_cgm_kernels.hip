@@ -568,4 +568,18 @@ void copy_words(const void *src, void *dst, size_t bytes, hipStream_t stream)
     KQ_HIP_CHECK(hipGetLastError());
 }
 
+__global__ void __launch_bounds__(256) k_set_int_list(const int *src, int *list, int *count, int n)
+{
+    for (int i = threadIdx.x; i < n; i += 256)
+        list[i] = src[i];
+    if (threadIdx.x == 0)
+        *count = n;
+}
+
+void set_int_list(const int *src, int *list, int *count, int n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_set_int_list, dim3(1), dim3(256), 0, stream, src, list, count, n);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace kq

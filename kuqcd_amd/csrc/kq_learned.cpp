@@ -285,9 +285,13 @@ FcgOut fcg_solve(const Geom &g, int q, AF af, AL al, PF pf, PL pl, LinkD df, Lin
         H.max_iter = net ? std::min(cfg.max_iter, H.iter + cfg.stall) : cfg.max_iter;
         upload();
         const int it0 = H.iter;
-        hipGraphExec_t gx = graph_for(net);
+        hipGraphExec_t gx = direct_launches() ? nullptr : graph_for(net);
         do {
-            KQ_HIP_CHECK(hipGraphLaunch(gx, st));
+            if (gx)
+                KQ_HIP_CHECK(hipGraphLaunch(gx, st));
+            else
+                for (int i = 0; i < cfg.chunk; ++i)
+                    enqueue(net);
             copy_words(S, sh.dev(), sizeof H, st);
             KQ_HIP_CHECK(hipStreamSynchronize(st));
         } while (!H.stop);

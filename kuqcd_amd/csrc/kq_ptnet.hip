@@ -109,8 +109,9 @@ __global__ void __launch_bounds__(kNT) k_pt_layer1(const PtLayerArgs a, LF fat, 
 {
     if (a.stop && *a.stop)
         return;
-    const uint32_t s = blockIdx.x * kNT + threadIdx.x;
-    if (s >= a.g.Vh)
+    bool valid;
+    const uint32_t s = tile_site<kNT>(a.g, blockIdx.x, gridDim.x, threadIdx.x, valid);
+    if (!valid)
         return;
     const SiteCoord sc = site_coords(a.g, s, a.qo);
     float2 acc[kPtHidden][3];
@@ -152,9 +153,8 @@ __global__ void __launch_bounds__(kNT) k_pt_layer2(const PtLayerArgs a, LF fat, 
 {
     if (a.stop && *a.stop)
         return;
-    const uint32_t s0 = blockIdx.x * kNT + threadIdx.x;
-    const bool valid = s0 < a.g.Vh;
-    const uint32_t s = valid ? s0 : a.g.Vh - 1;
+    bool valid;
+    const uint32_t s = tile_site<kNT>(a.g, blockIdx.x, gridDim.x, threadIdx.x, valid);
     const SiteCoord sc = site_coords(a.g, s, a.qo);
     float2 acc[3];
     const float2 w0 = unpack_w(a.w.w0);
@@ -219,9 +219,8 @@ __global__ void __launch_bounds__(kNT) k_pt_wgrad(Geom g, int qo, LF fat, LL lng
                                                   const float2 *b, double2 *part)
 {
     __shared__ double lds[kNT / kWave][kPtHidden * kPtTaps * 2];
-    const uint32_t s0 = blockIdx.x * kNT + threadIdx.x;
-    const bool valid = s0 < g.Vh;
-    const uint32_t s = valid ? s0 : g.Vh - 1;
+    bool valid;
+    const uint32_t s = tile_site<kNT>(g, blockIdx.x, gridDim.x, threadIdx.x, valid);
     const SiteCoord sc = site_coords(g, s, qo);
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     float2 bb[kPtHidden][3];

@@ -91,6 +91,18 @@ private:
     hipStream_t stream_ = nullptr;
 };
 
+// KQ_DIRECT_LAUNCH=1 makes the solvers launch their iteration kernels one by one
+// instead of replaying hipGraphs (same kernels, grids and order; for tools that
+// trace launches)
+inline bool direct_launches()
+{
+    static const bool on = [] {
+        const char *e = getenv("KQ_DIRECT_LAUNCH");
+        return e != nullptr && e[0] == '1';
+    }();
+    return on;
+}
+
 class DevBuf {
 public:
     DevBuf() = default;
@@ -219,6 +231,7 @@ public:
         x_.alloc(sizeof(V2) * pool_vecs * n3_);
         state_.alloc(sizeof(CgmState<S>) * max_sys);
         state_h_.alloc(max_sys);
+        list_stage_.alloc((size_t)kStages * max_sys);
         std::memset(state_h_.get(), 0, sizeof(CgmState<S>) * max_sys);
         for (int i = 0; i < max_sys; ++i)
             state_h_[i].stop = 1;  // unused slots never run
@@ -397,8 +410,8 @@ public:
             if (!restart.empty()) {
                 set_list(restart);
                 copy_r_to_p<S, VR>(state_.as<CgmState<S>>(), vec(r_), p_.as<VR>(), off_.as<size_t>(), list(),
-                                   list_n(), max_sys_, st_);
-                cgm_restart_list<S>(state_.as<CgmState<S>>(), list(), list_n(), max_sys_, st_);
+                                   list_n(), list_grid(), st_);
+                cgm_restart_list<S>(state_.as<CgmState<S>>(), list(), list_n(), list_grid(), st_);
                 restarts_ += (int)restart.size();
                 download_states();
                 if (dbg)
@@ -463,17 +476,32 @@ private:
     {
         copy_words(state_.as<void>(), state_h_.dev(), sizeof(CgmState<S>) * max_sys_, st_);
         KQ_HIP_CHECK(hipStreamSynchronize(st_));
+        stage_uses_ = 0;
     }
     void upload_states() { copy_words(state_h_.dev(), state_.as<void>(), sizeof(CgmState<S>) * max_sys_, st_); }
     // upload a list of systems for the list-driven kernels (blocking)
+    // Upload a list of systems for the list-driven kernels without waiting: the host
+    // writes it into one of kStages slots of mapped pinned memory and a kernel copies
+    // it. A slot is reused only after the stream has been synchronised since its last
+    // use (download_states synchronises; after kStages lists without one, wait here).
+    static constexpr int kStages = 4;
     void set_list(const std::vector<int> &l)
     {
         const int n = (int)l.size();
-        if (n > 0)
-            KQ_HIP_CHECK(hipMemcpyAsync(list(), l.data(), sizeof(int) * n, hipMemcpyHostToDevice, st_));
-        KQ_HIP_CHECK(hipMemcpyAsync(list_n(), &n, sizeof(int), hipMemcpyHostToDevice, st_));
-        KQ_HIP_CHECK(hipStreamSynchronize(st_));
+        if (n > max_sys_)
+            throw std::logic_error("set_list: more systems than the runner holds");
+        if (stage_uses_ >= kStages) {
+            KQ_HIP_CHECK(hipStreamSynchronize(st_));
+            stage_uses_ = 0;
+        }
+        const size_t off = (size_t)stage_slot_ * max_sys_;
+        std::copy(l.begin(), l.end(), list_stage_.get() + off);
+        set_int_list(list_stage_.dev() + off, list(), list_n(), n, st_);
+        stage_slot_ = (stage_slot_ + 1) % kStages;
+        ++stage_uses_;
+        list_size_ = std::max(n, 1);
     }
+    int list_grid() const { return list_size_; }
 
     // ax64 = A_0 x_hi[.][0] for the n listed systems (FP64 links, read once for up to 4 systems)
     void apply_a64_x0(int n)
@@ -485,7 +513,7 @@ private:
         a.lng = lng64_;
         a.rlist = list();
         a.nact = list_n();
-        a.nrhs = max_sys_;
+        a.nrhs = n;
         Vec<double> x0{x_hi_.as<double2>(), g_.Vh, 0, off_.as<size_t>()};
         Vec<double> t{tmp64_.as<double2>(), g_.Vh, n3_};
         Vec<double> ax{ax64_.as<double2>(), g_.Vh, n3_};
@@ -522,15 +550,15 @@ private:
                 apply_a64_x0((int)pending_x0_.size());
             }
             set_list(pending_);
-            rel_residual<VR>(b_hi_.as<double>(), ax64_.as<double>(), vec(r_), list(), list_n(), max_sys_,
+            rel_residual<VR>(b_hi_.as<double>(), ax64_.as<double>(), vec(r_), list(), list_n(), list_grid(),
                              part_b_.as<double>(), nblk_, st_);
         } else {
             set_list(pending_);
-            vec_dot_partial<VR>(vec(r_), vec(r_), max_sys_, part_b_.as<double>(), nblk_, st_, list(), list_n());
+            vec_dot_partial<VR>(vec(r_), vec(r_), list_grid(), part_b_.as<double>(), nblk_, st_, list(), list_n());
         }
         copy_r_to_p<S, VR>(state_.as<CgmState<S>>(), vec(r_), p_.as<VR>(), off_.as<size_t>(), list(), list_n(),
-                           max_sys_, st_);
-        cgm_init_list<S>(state_.as<CgmState<S>>(), list(), list_n(), max_sys_, part_b_.as<double>(), nblk_, st_);
+                           list_grid(), st_);
+        cgm_init_list<S>(state_.as<CgmState<S>>(), list(), list_n(), list_grid(), part_b_.as<double>(), nblk_, st_);
         pending_.clear();
         pending_x0_.clear();
         download_states();
@@ -542,16 +570,16 @@ private:
         if (!fold.empty()) {
             set_list(fold);
             rel_fold<S, VR>(state_.as<CgmState<S>>(), x_hi_.as<double>(), x_.as<VR>(), off_.as<size_t>(), g_.Vh,
-                            list(), list_n(), max_sys_, st_);
+                            list(), list_n(), list_grid(), st_);
             for (int sys : fold)
                 last_fold_[sys] = state_h_[sys].stop ? 0 : std::max(0, state_h_[sys].maxTerm);
         }
         if (!rel.empty()) {
             set_list(rel);
             apply_a64_x0((int)rel.size());
-            rel_residual<VR>(b_hi_.as<double>(), ax64_.as<double>(), vec(r_), list(), list_n(), max_sys_,
+            rel_residual<VR>(b_hi_.as<double>(), ax64_.as<double>(), vec(r_), list(), list_n(), list_grid(),
                              part_b_.as<double>(), nblk_, st_);
-            rel_set_rr<S>(state_.as<CgmState<S>>(), list(), list_n(), max_sys_, part_b_.as<double>(), nblk_, st_);
+            rel_set_rr<S>(state_.as<CgmState<S>>(), list(), list_n(), list_grid(), part_b_.as<double>(), nblk_, st_);
             reliable_updates_ += 1;
         }
         download_states();  // keep the host copy current (rr, r2max set on the device)
@@ -590,17 +618,6 @@ private:
         cgm_xp_update<S, VR, VR>(st, x_.as<VR>(), p_.as<VR>(), vec(r_), L, st_);
     }
 
-    // KQ_DIRECT_LAUNCH=1 launches the iteration's kernels one by one instead of
-    // replaying graphs (same kernels and grids; for tools that trace launches)
-    static bool direct_launches()
-    {
-        static const bool on = [] {
-            const char *e = getenv("KQ_DIRECT_LAUNCH");
-            return e != nullptr && e[0] == '1';
-        }();
-        return on;
-    }
-
     // graph of one iteration for nlive live systems: right-hand-side blocking nr
     // from the tuning table and ceil(nlive / nr) groups (no grid slots for
     // systems that cannot be live during the chunk)
@@ -632,6 +649,8 @@ private:
     uint32_t nblk_ = 0, nblk_dsl_ = 0;
     DevBuf r_, s_, tmp_, p_, x_, state_, part_a_, part_b_, ints_, sigma_, off_;
     Pinned<CgmState<S>> state_h_;
+    Pinned<int> list_stage_;  // kStages slots of max_sys_ system indices
+    int stage_slot_ = 0, stage_uses_ = 0, list_size_ = 1;
     std::vector<size_t> off_h_;
     std::vector<double> sig_h_;
     int nsys_ = 0;
