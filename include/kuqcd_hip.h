@@ -165,6 +165,10 @@ KQ_API KuQCDStatus kq_ptnet_norms(double nu[2]);
  * link_format: KQ_MODE_MIXED_FP32 or KQ_MODE_MIXED_FP16 */
 KQ_API KuQCDStatus kq_ptnet_forward(const KqPtWeights *w, int parity, int link_format, const void *r, void *z,
                                     void *h);
+/* the same with the hidden channels passed from layer 1 to layer 2 as bfloat16 (round to nearest
+ * even), as the flexible CG applies the network; h (may be NULL) receives the rounded channels */
+KQ_API KuQCDStatus kq_ptnet_forward_h16(const KqPtWeights *w, int parity, int link_format, const void *r, void *z,
+                                        void *h);
 /* loss L(r, e) for A = sigma - D D and its gradient dL/dconj(W) with respect to the weights as
  * stored in KqPtWeights (258 doubles: w1, w2, w0 as re, im pairs); FP32 links and vectors */
 KQ_API KuQCDStatus kq_ptnet_loss_grad(const KqPtWeights *w, int parity, double sigma, const void *r, const void *e,

@@ -205,6 +205,7 @@ _PROTOS = {
     "kq_ptnet_get_weights": (C.c_int, [C.POINTER(KqPtWeights)]),
     "kq_ptnet_norms": (C.c_int, [C.POINTER(C.c_double)]),
     "kq_ptnet_forward": (C.c_int, [C.POINTER(KqPtWeights), C.c_int, C.c_int, _VP, _VP, _VP]),
+    "kq_ptnet_forward_h16": (C.c_int, [C.POINTER(KqPtWeights), C.c_int, C.c_int, _VP, _VP, _VP]),
     "kq_ptnet_loss_grad": (C.c_int, [C.POINTER(KqPtWeights), C.c_int, C.c_double, _VP, _VP,
                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "kq_learned_collect": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_uint64, C.c_double, C.c_double,

@@ -254,11 +254,13 @@ def ptnet_norms() -> tuple[float, float]:
 
 
 def ptnet_forward(w: L.KqPtWeights, r: np.ndarray, parity: int = 0, link_format: int = L.KQ_MODE_MIXED_FP32,
-                  hidden: bool = False):
-    """z = M r (and the hidden channels, shape (4, Vh, 3, 2), if hidden)."""
+                  hidden: bool = False, hidden16: bool = False):
+    """z = M r (and the hidden channels, shape (4, Vh, 3, 2), if hidden). hidden16: the channels pass
+    between the layers as bfloat16, as in the flexible CG (kq_ptnet_forward_h16)."""
     z = np.empty_like(r)
     h = np.empty((4,) + r.shape, dtype=np.float64) if hidden else None
-    st = L.lib().kq_ptnet_forward(C.byref(w), parity, link_format, _ptr(r), _ptr(z), _ptr(h))
+    fn = L.lib().kq_ptnet_forward_h16 if hidden16 else L.lib().kq_ptnet_forward
+    st = fn(C.byref(w), parity, link_format, _ptr(r), _ptr(z), _ptr(h))
     if st != L.KUQCD_SUCCESS:
         raise RuntimeError(f"kq_ptnet_forward: {st} {L.last_error()}")
     return (z, h) if hidden else z

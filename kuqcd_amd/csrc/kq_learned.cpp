@@ -186,7 +186,9 @@ FcgOut fcg_solve(const Geom &g, int q, AF af, AL al, PF pf, PL pl, LinkD df, Lin
     const OpA<AF, AL, float> A{g, q, af, al, cfg.nt_a};
     const OpA<LinkD, LinkD, double> D64{g, q, df, dl, cfg.nt_d};
     const size_t fb = sizeof(float2) * n3, db = sizeof(double2) * n3;
-    DevBuf vr(fb), vz(cfg.learned ? fb : 0), vp(fb), vq(fb), vt(fb), vx(fb), vh(cfg.learned ? 4 * fb : 0);
+    DevBuf vr(fb), vz(cfg.learned ? fb : 0), vp(fb), vq(fb), vt(fb), vx(fb);
+    // hidden channels between the two network layers as bfloat16 (48 B per site)
+    DevBuf vh16(cfg.learned ? sizeof(kq_v4u) * n3 : 0);
     DevBuf ax64(db), t64(db), r64(hook ? db : 0), xbest(cfg.learned ? db : 0);
     DevBuf part_qz(sizeof(double) * 2 * std::max(nbv, nbl)), part_pr(sizeof(double) * 2 * nbv),
         part_pq(sizeof(double) * std::max(A.nblk(), D64.nblk())), part_rr(sizeof(double) * nbv);
@@ -219,8 +221,8 @@ FcgOut fcg_solve(const Geom &g, int q, AF af, AL al, PF pf, PL pl, LinkD df, Lin
 
     auto enqueue = [&](bool net) {
         if (net) {
-            pt_forward<PF, PL>(g, q, pf, pl, cfg.wf, vr.as<float2>(), vh.as<float2>(), vz.as<float2>(), st,
-                               vq.as<float2>(), part_qz.as<double>(), &S->stop);
+            pt_forward<PF, PL>(g, q, pf, pl, cfg.wf, vr.as<float2>(), nullptr, vz.as<float2>(), st,
+                               vq.as<float2>(), part_qz.as<double>(), &S->stop, vh16.as<kq_v4u>());
             fcg_beta(S, part_qz.as<double>(), nbl, st);
             fcg_p_update(S, vz.as<float2>(), vp.as<float2>(), vr.as<float2>(), n3, part_pr.as<double>(), st);
         } else {
@@ -616,7 +618,8 @@ KuQCDStatus kq_ptnet_norms(double nu[2])
     });
 }
 
-KuQCDStatus kq_ptnet_forward(const KqPtWeights *w, int parity, int link_format, const void *r, void *z, void *h)
+static KuQCDStatus ptnet_forward(const KqPtWeights *w, int parity, int link_format, const void *r, void *z, void *h,
+                                 bool hidden16)
 {
     if (!w || !r || !z) {
         set_error("w, r and z must not be nullptr");
@@ -633,10 +636,14 @@ KuQCDStatus kq_ptnet_forward(const KqPtWeights *w, int parity, int link_format, 
         const size_t n3 = (size_t)3 * g.Vh;
         const PtWeightsBF wf = pt_pack(from_abi(*w), slot_norms(slot));
         DevBuf vr(sizeof(float2) * n3), vz(sizeof(float2) * n3), vh(sizeof(float2) * 4 * n3), stage;
+        DevBuf vh16(hidden16 ? sizeof(kq_v4u) * n3 : 0);
         upload_vec<double, float>(r, Vec<float>{vr.as<float2>(), g.Vh, n3}, 0, stage, c.stream);
         with_format(slot, link_format, [&](auto fat, auto lng) {
-            pt_forward(g, parity, fat, lng, wf, vr.as<float2>(), vh.as<float2>(), vz.as<float2>(), c.stream);
+            pt_forward(g, parity, fat, lng, wf, vr.as<float2>(), hidden16 ? nullptr : vh.as<float2>(),
+                       vz.as<float2>(), c.stream, nullptr, nullptr, nullptr, hidden16 ? vh16.as<kq_v4u>() : nullptr);
         });
+        if (hidden16 && h)
+            pt_unpack_hidden(vh16.as<kq_v4u>(), vh.as<float2>(), g.Vh, c.stream);
         download_vec<float, double>(Vec<float>{vz.as<float2>(), g.Vh, n3}, 0, z, stage, c.stream);
         if (h)
             for (int ch = 0; ch < kPtHidden; ++ch)
@@ -644,6 +651,16 @@ KuQCDStatus kq_ptnet_forward(const KqPtWeights *w, int parity, int link_format, 
                                             static_cast<double *>(h) + (size_t)ch * 2 * n3, stage, c.stream);
         return KUQCD_SUCCESS;
     });
+}
+
+KuQCDStatus kq_ptnet_forward(const KqPtWeights *w, int parity, int link_format, const void *r, void *z, void *h)
+{
+    return ptnet_forward(w, parity, link_format, r, z, h, false);
+}
+
+KuQCDStatus kq_ptnet_forward_h16(const KqPtWeights *w, int parity, int link_format, const void *r, void *z, void *h)
+{
+    return ptnet_forward(w, parity, link_format, r, z, h, true);
 }
 
 KuQCDStatus kq_ptnet_loss_grad(const KqPtWeights *w, int parity, double sigma, const void *r, const void *e,

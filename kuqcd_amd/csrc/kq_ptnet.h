@@ -65,9 +65,15 @@ PtWeights pt_effective(const PtWeights &w, const PtNorms &nu);
 // If dotq is given, also writes per-block partial sums of <dotq, z> (complex) to
 // part[2*blk], part[2*blk+1], with pt_layer_nblk(Vh) blocks. If stop is given, the
 // kernels do nothing while *stop != 0.
+// If h16 is given, the hidden channels pass from layer 1 to layer 2 as bfloat16 in h16
+// (3*Vh entries of 16 B: colour k of site s holds the 4 channels' (re, im) bfloat16 pairs
+// at h16[k*Vh + s]); h (FP32) is then written only if not null. Otherwise layer 2 reads h.
 template <class LF, class LL>
 void pt_forward(const Geom &g, int q, LF fat, LL lng, const PtWeightsBF &w, const float2 *r, float2 *h, float2 *z,
-                hipStream_t st, const float2 *dotq = nullptr, double *part = nullptr, const int *stop = nullptr);
+                hipStream_t st, const float2 *dotq = nullptr, double *part = nullptr, const int *stop = nullptr,
+                kq_v4u *h16 = nullptr);
+// FP32 copy of bfloat16 hidden channels (h16 layout above -> 4 channels, channel stride 3*Vh)
+void pt_unpack_hidden(const kq_v4u *h16, float2 *h, uint32_t Vh, hipStream_t st);
 // Backward to the hidden layer: gh_c = sum_t conj(W2[c][t]) T_t^+ g  (parity 1-q); wb from pt_pack_backward
 template <class LF, class LL>
 void pt_backward_hidden(const Geom &g, int q, LF fat, LL lng, const PtWeightsBF &wb, const float2 *gz, float2 *gh,

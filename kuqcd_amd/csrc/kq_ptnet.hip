@@ -24,6 +24,15 @@ __device__ __forceinline__ float2 unpack_w(uint32_t u)
     return mk<float2>(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
 }
 
+// (re, im) -> bfloat16 pair in unpack_w's format, round to nearest even (finite values)
+__device__ __forceinline__ uint32_t pack_bf16(float2 v)
+{
+    uint32_t a = __float_as_uint(v.x), b = __float_as_uint(v.y);
+    a += 0x7fffu + ((a >> 16) & 1u);
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return (a >> 16) | (b & 0xffff0000u);
+}
+
 // acc = L(s) x  or  acc = L(n)^+ x, for the tap's link format
 template <class LK, bool NEG, typename R>
 __device__ __forceinline__ void tap_mul(const LK &lk, int qo, int mu, uint32_t s, uint32_t n, const float2 x[3],
@@ -85,6 +94,27 @@ __device__ __forceinline__ void tap_comb(const Geom &g, int qo, const LK &lk, ui
         v[c] = mk<float2>(acc[2 * c], acc[2 * c + 1]);
 }
 
+// the same with the channels as bfloat16 (16 B per colour: 4 channels' (re, im) pairs)
+template <class LK, int MU, int H, bool NEG>
+__device__ __forceinline__ void tap_comb16(const Geom &g, int qo, const LK &lk, uint32_t s, const SiteCoord &sc,
+                                           const kq_v4u *in, const float2 wt[kPtHidden], float2 v[3])
+{
+    const uint32_t n = nb_index<MU>(g, s, sc, NEG ? -H : H);
+    float2 x[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const kq_v4u u = in[(size_t)c * g.Vh + n];
+        const float2 p0 = cmulf(wt[0], unpack_w(u.x)), p1 = cmulf(wt[1], unpack_w(u.y));
+        const float2 p2 = cmulf(wt[2], unpack_w(u.z)), p3 = cmulf(wt[3], unpack_w(u.w));
+        x[c] = mk<float2>((p0.x + p1.x) + (p2.x + p3.x), (p0.y + p1.y) + (p2.y + p3.y));
+    }
+    float acc[6];
+    tap_mul<LK, NEG>(lk, qo, MU, s, n, x, acc);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        v[c] = mk<float2>(acc[2 * c], acc[2 * c + 1]);
+}
+
 // the 16 taps as (t, mu, hop, negative)
 #define KQ_PT_FOR_TAPS(X)                                                                                       \
     X(0, 0, 1, false) X(1, 0, 3, false) X(2, 0, 1, true) X(3, 0, 3, true) X(4, 1, 1, false) X(5, 1, 3, false)   \
@@ -95,7 +125,9 @@ struct PtLayerArgs {
     Geom g;
     int qo;              // output parity
     const float2 *in;    // input field(s) on parity 1-qo, channel stride 3*Vh
-    float2 *out;         // output field(s) on parity qo, channel stride 3*Vh
+    float2 *out;         // output field(s) on parity qo, channel stride 3*Vh (may be null with out16)
+    kq_v4u *out16;       // layer 1: the outputs also as bfloat16 (pt_forward's h16 layout); may be null
+    const kq_v4u *in16;  // layer 2: read the channels from here (bfloat16) instead of in; may be null
     const float2 *skip;  // layer 2: w0 * skip(x) added (parity qo); may be null
     const float2 *dotq;  // layer 2: per-block partials of <dotq, out> (may be null)
     double *part;        // [2*nblk]
@@ -140,15 +172,29 @@ __global__ void __launch_bounds__(kNT) k_pt_layer1(const PtLayerArgs a, LF fat, 
     }
     KQ_PT_FOR_TAPS(KQ_TAP1)
 #undef KQ_TAP1
+    if (a.out) {
 #pragma unroll
-    for (int c = 0; c < kPtHidden; ++c)
+        for (int c = 0; c < kPtHidden; ++c)
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            a.out[((size_t)c * 3 + k) * a.g.Vh + s] = acc[c][k];
+            for (int k = 0; k < 3; ++k)
+                a.out[((size_t)c * 3 + k) * a.g.Vh + s] = acc[c][k];
+    }
+    if (a.out16) {
+        static_assert(kPtHidden == 4, "the bfloat16 hidden layout holds 4 channels");
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            kq_v4u u;
+            u.x = pack_bf16(acc[0][k]);
+            u.y = pack_bf16(acc[1][k]);
+            u.z = pack_bf16(acc[2][k]);
+            u.w = pack_bf16(acc[3][k]);
+            a.out16[(size_t)k * a.g.Vh + s] = u;
+        }
+    }
 }
 
-// layer-2 form: out(x) = w0 skip(x) + sum_t T_t( sum_c w2[c][t] in_c )(x)
-template <class LF, class LL>
+// layer-2 form: out(x) = w0 skip(x) + sum_t T_t( sum_c w2[c][t] in_c )(x); H16: channels from in16
+template <class LF, class LL, bool H16>
 __global__ void __launch_bounds__(kNT) k_pt_layer2(const PtLayerArgs a, LF fat, LL lng)
 {
     if (a.stop && *a.stop)
@@ -165,7 +211,12 @@ __global__ void __launch_bounds__(kNT) k_pt_layer2(const PtLayerArgs a, LF fat, 
     {                                                                              \
         float2 wt[kPtHidden], v[3];                                                \
         _Pragma("unroll") for (int c = 0; c < kPtHidden; ++c) wt[c] = unpack_w(a.w.w2[c][T]); \
-        if (H == 1)                                                                \
+        if constexpr (H16) {                                                       \
+            if (H == 1)                                                            \
+                tap_comb16<LF, MU, H, NEG>(a.g, a.qo, fat, s, sc, a.in16, wt, v);  \
+            else                                                                   \
+                tap_comb16<LL, MU, H, NEG>(a.g, a.qo, lng, s, sc, a.in16, wt, v);  \
+        } else if (H == 1)                                                         \
             tap_comb<LF, MU, H, NEG>(a.g, a.qo, fat, s, sc, a.in, wt, v);          \
         else                                                                       \
             tap_comb<LL, MU, H, NEG>(a.g, a.qo, lng, s, sc, a.in, wt, v);          \
@@ -363,15 +414,36 @@ __global__ void __launch_bounds__(kNT) k_vd_sumsq(const double *x, size_t n, dou
     }
 }
 
+__global__ void __launch_bounds__(kNT) k_unpack_hidden(const kq_v4u *h16, float2 *h, uint32_t Vh)
+{
+    const size_t i = (size_t)blockIdx.x * kNT + threadIdx.x;  // colour-major entry k*Vh + s
+    if (i >= (size_t)3 * Vh)
+        return;
+    const kq_v4u u = h16[i];
+    const size_t n3 = (size_t)3 * Vh;
+    h[i] = unpack_w(u.x);
+    h[n3 + i] = unpack_w(u.y);
+    h[2 * n3 + i] = unpack_w(u.z);
+    h[3 * n3 + i] = unpack_w(u.w);
+}
+
 inline unsigned grid_of(size_t n) { return (unsigned)((n + kNT - 1) / kNT); }
 
 }  // namespace
+
+void pt_unpack_hidden(const kq_v4u *h16, float2 *h, uint32_t Vh, hipStream_t st)
+{
+    if (Vh == 0)
+        return;
+    hipLaunchKernelGGL(k_unpack_hidden, dim3(grid_of((size_t)3 * Vh)), dim3(kNT), 0, st, h16, h, Vh);
+    KQ_HIP_CHECK(hipGetLastError());
+}
 
 uint32_t pt_layer_nblk(uint32_t Vh) { return (Vh + kNT - 1) / kNT; }
 
 template <class LF, class LL>
 void pt_forward(const Geom &g, int q, LF fat, LL lng, const PtWeightsBF &w, const float2 *r, float2 *h, float2 *z,
-                hipStream_t st, const float2 *dotq, double *part, const int *stop)
+                hipStream_t st, const float2 *dotq, double *part, const int *stop, kq_v4u *h16)
 {
     if (g.Vh == 0)
         return;
@@ -383,15 +455,21 @@ void pt_forward(const Geom &g, int q, LF fat, LL lng, const PtWeightsBF &w, cons
     a.qo = 1 - q;
     a.in = r;
     a.out = h;
+    a.out16 = h16;
     hipLaunchKernelGGL((k_pt_layer1<LF, LL>), dim3(nb), dim3(kNT), 0, st, a, fat, lng);
     KQ_HIP_CHECK(hipGetLastError());
     a.qo = q;
     a.in = h;
+    a.in16 = h16;
+    a.out16 = nullptr;
     a.out = z;
     a.skip = r;
     a.dotq = dotq;
     a.part = part;
-    hipLaunchKernelGGL((k_pt_layer2<LF, LL>), dim3(nb), dim3(kNT), 0, st, a, fat, lng);
+    if (h16)
+        hipLaunchKernelGGL((k_pt_layer2<LF, LL, true>), dim3(nb), dim3(kNT), 0, st, a, fat, lng);
+    else
+        hipLaunchKernelGGL((k_pt_layer2<LF, LL, false>), dim3(nb), dim3(kNT), 0, st, a, fat, lng);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
@@ -590,7 +668,7 @@ double vd_sumsq(const double *x, size_t n, hipStream_t st)
 
 #define KQ_PT_INST(LF, LL)                                                                                          \
     template void pt_forward<LF, LL>(const Geom &, int, LF, LL, const PtWeightsBF &, const float2 *, float2 *,       \
-                                     float2 *, hipStream_t, const float2 *, double *, const int *);                 \
+                                     float2 *, hipStream_t, const float2 *, double *, const int *, kq_v4u *);       \
     template void pt_backward_hidden<LF, LL>(const Geom &, int, LF, LL, const PtWeightsBF &, const float2 *,         \
                                              float2 *, hipStream_t);                                                \
     template void pt_weight_grads<LF, LL>(const Geom &, int, LF, LL, const float2 *, const float2 *,                 \

@@ -159,10 +159,20 @@ def forward(G: Geometry, U1, U3, W1e, W2e, w0e, q: int, r: np.ndarray):
     """h (4, Vh, 3) on parity 1-q and z (Vh, 3) on parity q, with effective weights."""
     T = [tap(G, U1, U3, 1 - q, t, r) for t in range(NT)]
     h = np.einsum("ct,tsi->csi", W1e, np.array(T))
+    return h, layer2(G, U1, U3, W2e, w0e, q, r, h)
+
+
+def layer2(G: Geometry, U1, U3, W2e, w0e, q: int, r: np.ndarray, h: np.ndarray):
+    """z = w0 r + sum_t T_t(sum_c W2[c][t] h_c) on parity q from given hidden channels h (4, Vh, 3)."""
     z = w0e * r
     for t in range(NT):
         z = z + tap(G, U1, U3, q, t, np.einsum("c,csi->si", W2e[:, t], h))
-    return h, z
+    return z
+
+
+def bf16_complex(h: np.ndarray) -> np.ndarray:
+    """complex values with real and imaginary parts rounded to bfloat16 (from float32)."""
+    return bf16(np.real(h)) + 1j * bf16(np.imag(h))
 
 
 def backward(G: Geometry, U1, U3, W2e, q: int, r, h, g):
@@ -211,6 +221,15 @@ def loss_grad(fat, lng, lat, q: int, sigma: float, W1, W2, w0, r, e, nu=None, ro
     gW1, gW2, gw0 = backward(G, U1, U3, W2e, q, r, h, g)
     s = np.array([1.0 / nu[1] if (t & 1) else 1.0 / nu[0] for t in range(NT)])
     return L, np.concatenate([(gW1 * s).ravel(), (gW2 * s).ravel(), [gw0]])
+
+
+def apply_layer2(fat, lng, lat, q: int, W2, w0, r, h, nu=None, round_bf16=True):
+    """z of the network's second layer from given hidden channels h (4, Vh, 3), master weights."""
+    U1, U3 = links(fat, lng, lat)
+    G = Geometry(lat)
+    nu = norms(U1, U3) if nu is None else nu
+    _, W2e, w0e = effective(np.zeros_like(np.asarray(W2)), W2, w0, nu, round_bf16)
+    return layer2(G, U1, U3, W2e, w0e, q, spinor(r), np.asarray(h, np.complex128))
 
 
 def apply_net(fat, lng, lat, q: int, W1, W2, w0, r, nu=None, round_bf16=True):

@@ -58,6 +58,32 @@ def test_forward_matches_oracle(gpu, lat, q):
 
 
 @pytest.mark.parametrize("lat,q", [((8, 8, 8, 8), 0), ((6, 4, 4, 6), 1)])
+def test_forward_bf16_hidden_matches_oracle(gpu, lat, q):
+    """The flexible CG's form of the network: hidden channels rounded to bfloat16 between the layers."""
+    fat, lng = _load_weak(gpu, lat, 1)
+    nu = gpu.ptnet_norms()
+    w = _weights(gpu, 5)
+    W1, W2, w0 = gpu.weights_to_array(w)
+    r = gpu.gen_source(lat, parity=q, seed=9)
+    h_o, z_o = P.apply_net(fat, lng, lat, q, W1, W2, w0, r, nu=nu)
+    z16, h16 = gpu.ptnet_forward(w, r, parity=q, link_format=gpu.KQ_MODE_MIXED_FP32, hidden=True, hidden16=True)
+    hc = np.array([_c(h16[c]).reshape(-1, 3) for c in range(4)])
+    # every hidden entry is a bfloat16 number within half a bfloat16 step (2^-8 relative) of the
+    # FP64 first layer (plus the FP32 accumulation error)
+    assert np.array_equal(P.bf16_complex(hc), hc)
+    for c in range(4):
+        d = hc[c] - h_o[c]
+        tiny = 1e-6 * np.abs(h_o[c]).max()
+        assert np.all(np.abs(d.real) <= 2.0 ** -8 * np.abs(h_o[c].real) + tiny)
+        assert np.all(np.abs(d.imag) <= 2.0 ** -8 * np.abs(h_o[c].imag) + tiny)
+    # the second layer on exactly these channels is the oracle's, to FP32 accuracy
+    z_o2 = P.apply_layer2(fat, lng, lat, q, W2, w0, r, hc, nu=nu)
+    assert _rel(_c(z16), z_o2) < 1e-5
+    # the rounding moves z by about the bfloat16 step, no more
+    assert 1e-5 < _rel(_c(z16), z_o) < 1e-2
+
+
+@pytest.mark.parametrize("lat,q", [((8, 8, 8, 8), 0), ((6, 4, 4, 6), 1)])
 def test_loss_and_gradient_match_oracle(gpu, lat, q):
     fat, lng = _load_weak(gpu, lat, 2)
     nu = gpu.ptnet_norms()
