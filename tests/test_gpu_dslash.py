@@ -50,7 +50,7 @@ def test_dslash_fp32_gauge(gpu, lat):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode,tol", [(2, 1e-5), (3, 3e-3)])
+@pytest.mark.parametrize("mode,tol", [(2, 1e-5), (3, 3e-3), (4, 5e-4)])
 @pytest.mark.parametrize("kind", [0, 1])
 def test_dslash_lower_precision_modes(gpu, mode, tol, kind):
     lat = (8, 8, 8, 8)
@@ -63,7 +63,7 @@ def test_dslash_lower_precision_modes(gpu, mode, tol, kind):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 def test_dslash_batched_equals_single(gpu, mode):
     lat = (8, 4, 4, 8)
     _setup(gpu, lat, gpu.KUQCD_DOUBLE)

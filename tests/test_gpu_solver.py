@@ -77,7 +77,7 @@ def test_cgm_fp64_matches_oracle_config1(gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 def test_cgm_modes_reach_tolerance(gpu, mode):
     kq = gpu
     lat = (8, 8, 8, 8)
