@@ -418,6 +418,7 @@ static void free_all_gauges()
     }
     c.tune.clear();
     c.tune_group.clear();
+    c.tune_cap.clear();
     DevPool::get().trim();
 }
 

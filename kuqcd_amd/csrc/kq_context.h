@@ -56,6 +56,8 @@ struct Context {
     // (mode, Vh) -> milliseconds of one group of NR = 1, 2, 4, 8 right-hand sides at block
     // size 64, 128, 256 (index 3 * log2(NR) + log2(nt / 64)); < 0 where not measured
     std::map<std::pair<int, uint32_t>, std::array<double, 12>> tune_group;
+    // (mode, Vh, nr, nt) -> occupancy cap of the hopping term (waves per SIMD, 0 = none)
+    std::map<std::tuple<int, uint32_t, int, int>, int> tune_cap;
 };
 
 Context &ctx();

@@ -28,9 +28,12 @@ template <class LF, class LL, typename SR> struct DslashArgs {
 };
 
 // Launch D over all sites of parity dst_parity for every right-hand side.
-// nr = right-hand sides handled per thread (1, 2, 4 or 8), nt = threads per block.
+// nr = right-hand sides handled per thread (1, 2, 4 or 8), nt = threads per block,
+// cap = at most this many wavefronts per SIMD (0: as many as the registers allow). The
+// cap is imposed with unused dynamic LDS; fewer resident waves keep the gathered
+// neighbour vectors of the batched terms in L2 (measured up to 19 % faster at 32^4).
 template <class LF, class LL, typename SR>
-void launch_dslash(DslashArgs<LF, LL, SR> a, int epi, int nr, int nt, hipStream_t st);
+void launch_dslash(DslashArgs<LF, LL, SR> a, int epi, int nr, int nt, hipStream_t st, int cap = 0);
 
 inline uint32_t dslash_nblk(uint32_t Vh, int nt)
 {

@@ -10,6 +10,7 @@
 // device links.
 #include "kq_dslash.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace kq {
@@ -154,20 +155,46 @@ __global__ void __launch_bounds__(NT) k_dslash(const DslashArgs<LF, LL, SR> a)
     }
 }
 
+// dynamic LDS bytes that let at most `cap` waves per SIMD (4 SIMDs per CU) be resident
+static size_t cap_lds_bytes(int cap, int nt)
+{
+    if (cap <= 0)
+        return 0;
+    static const int lds_cu = [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || v <= 0)
+            v = 65536;
+        return v;
+    }();
+    static const int lds_block = [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 65536;
+        return v;
+    }();
+    const int blocks = std::max(1, cap * 4 / std::max(1, nt / kWave));
+    // a little under lds_cu / blocks, so that `blocks` fit and blocks + 1 do not
+    const int bytes = lds_cu / blocks - 256;
+    return (size_t)std::max(0, std::min(bytes, lds_block - 256));
+}
+
 template <class LF, class LL, typename SR, int NR, int EPI>
-static void launch_nt(const DslashArgs<LF, LL, SR> &a, int nt, hipStream_t st)
+static void launch_nt(const DslashArgs<LF, LL, SR> &a, int nt, int cap, hipStream_t st)
 {
     const int nact_max = a.nrhs;
     dim3 grid((a.g.Vh + nt - 1) / nt, (nact_max + NR - 1) / NR);
+    const size_t lds = cap_lds_bytes(cap, nt);
     switch (nt) {
     case 64:
-        hipLaunchKernelGGL((k_dslash<LF, LL, SR, NR, EPI, 64>), grid, dim3(64), 0, st, a);
+        hipLaunchKernelGGL((k_dslash<LF, LL, SR, NR, EPI, 64>), grid, dim3(64), lds, st, a);
         break;
     case 128:
-        hipLaunchKernelGGL((k_dslash<LF, LL, SR, NR, EPI, 128>), grid, dim3(128), 0, st, a);
+        hipLaunchKernelGGL((k_dslash<LF, LL, SR, NR, EPI, 128>), grid, dim3(128), lds, st, a);
         break;
     case 256:
-        hipLaunchKernelGGL((k_dslash<LF, LL, SR, NR, EPI, 256>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_dslash<LF, LL, SR, NR, EPI, 256>), grid, dim3(256), lds, st, a);
         break;
     default:
         throw std::runtime_error("dslash: unsupported block size " + std::to_string(nt));
@@ -176,44 +203,44 @@ static void launch_nt(const DslashArgs<LF, LL, SR> &a, int nt, hipStream_t st)
 }
 
 template <class LF, class LL, typename SR, int NR>
-static void launch_epi(const DslashArgs<LF, LL, SR> &a, int epi, int nt, hipStream_t st)
+static void launch_epi(const DslashArgs<LF, LL, SR> &a, int epi, int nt, int cap, hipStream_t st)
 {
     if (epi == EPI_STORE)
-        launch_nt<LF, LL, SR, NR, EPI_STORE>(a, nt, st);
+        launch_nt<LF, LL, SR, NR, EPI_STORE>(a, nt, cap, st);
     else
-        launch_nt<LF, LL, SR, NR, EPI_AXMY_DOT>(a, nt, st);
+        launch_nt<LF, LL, SR, NR, EPI_AXMY_DOT>(a, nt, cap, st);
 }
 
 template <class LF, class LL, typename SR>
-void launch_dslash(DslashArgs<LF, LL, SR> a, int epi, int nr, int nt, hipStream_t st)
+void launch_dslash(DslashArgs<LF, LL, SR> a, int epi, int nr, int nt, hipStream_t st, int cap)
 {
     if (a.g.Vh == 0 || a.nrhs <= 0)
         return;
     a.nblk = (a.g.Vh + nt - 1) / nt;
     switch (nr) {
     case 1:
-        launch_epi<LF, LL, SR, 1>(a, epi, nt, st);
+        launch_epi<LF, LL, SR, 1>(a, epi, nt, cap, st);
         break;
     case 2:
-        launch_epi<LF, LL, SR, 2>(a, epi, nt, st);
+        launch_epi<LF, LL, SR, 2>(a, epi, nt, cap, st);
         break;
     case 4:
-        launch_epi<LF, LL, SR, 4>(a, epi, nt, st);
+        launch_epi<LF, LL, SR, 4>(a, epi, nt, cap, st);
         break;
     case 8:
-        launch_epi<LF, LL, SR, 8>(a, epi, nt, st);
+        launch_epi<LF, LL, SR, 8>(a, epi, nt, cap, st);
         break;
     default:
         throw std::runtime_error("dslash: unsupported RHS blocking " + std::to_string(nr));
     }
 }
 
-template void launch_dslash<LinkD, LinkD, double>(DslashArgs<LinkD, LinkD, double>, int, int, int, hipStream_t);
-template void launch_dslash<LinkF, LinkF, float>(DslashArgs<LinkF, LinkF, float>, int, int, int, hipStream_t);
-template void launch_dslash<LinkF, LinkF12, float>(DslashArgs<LinkF, LinkF12, float>, int, int, int, hipStream_t);
-template void launch_dslash<LinkH, LinkH, float>(DslashArgs<LinkH, LinkH, float>, int, int, int, hipStream_t);
-template void launch_dslash<LinkF, LinkH, float>(DslashArgs<LinkF, LinkH, float>, int, int, int, hipStream_t);
-template void launch_dslash<LinkF, LinkH12, float>(DslashArgs<LinkF, LinkH12, float>, int, int, int, hipStream_t);
-template void launch_dslash<LinkH, LinkH12, float>(DslashArgs<LinkH, LinkH12, float>, int, int, int, hipStream_t);
+template void launch_dslash<LinkD, LinkD, double>(DslashArgs<LinkD, LinkD, double>, int, int, int, hipStream_t, int);
+template void launch_dslash<LinkF, LinkF, float>(DslashArgs<LinkF, LinkF, float>, int, int, int, hipStream_t, int);
+template void launch_dslash<LinkF, LinkF12, float>(DslashArgs<LinkF, LinkF12, float>, int, int, int, hipStream_t, int);
+template void launch_dslash<LinkH, LinkH, float>(DslashArgs<LinkH, LinkH, float>, int, int, int, hipStream_t, int);
+template void launch_dslash<LinkF, LinkH, float>(DslashArgs<LinkF, LinkH, float>, int, int, int, hipStream_t, int);
+template void launch_dslash<LinkF, LinkH12, float>(DslashArgs<LinkF, LinkH12, float>, int, int, int, hipStream_t, int);
+template void launch_dslash<LinkH, LinkH12, float>(DslashArgs<LinkH, LinkH12, float>, int, int, int, hipStream_t, int);
 
 }  // namespace kq

@@ -117,6 +117,7 @@ template <class LF, class LL, typename R> struct OpA {
     LF fat;
     LL lng;
     int nt;
+    int cap = 0;  // occupancy cap of the hopping term (waves per SIMD, 0 = none)
     uint32_t nblk() const { return dslash_nblk(g.Vh, nt); }
     // dst = sigma src - D D src (parity q), tmp on parity 1-q; part[nblk] = partials of Re<src, dst>
     void apply(const V2 *src, V2 *tmp, V2 *dst, const double *sigma, double *part, const int *run,
@@ -131,14 +132,14 @@ template <class LF, class LL, typename R> struct OpA {
         a.dst_parity = 1 - q;
         a.src = Vec<R>{const_cast<V2 *>(src), g.Vh, 0};
         a.dst = Vec<R>{tmp, g.Vh, 0};
-        launch_dslash(a, EPI_STORE, 1, nt, st);
+        launch_dslash(a, EPI_STORE, 1, nt, st, cap);
         a.dst_parity = q;
         a.src = Vec<R>{tmp, g.Vh, 0};
         a.aux = Vec<R>{const_cast<V2 *>(src), g.Vh, 0};
         a.dst = Vec<R>{dst, g.Vh, 0};
         a.sigma = sigma;
         a.partial = part;
-        launch_dslash(a, EPI_AXMY_DOT, 1, nt, st);
+        launch_dslash(a, EPI_AXMY_DOT, 1, nt, st, cap);
     }
 };
 
@@ -166,6 +167,7 @@ struct FcgCfg {
     bool learned = false;
     PtWeightsBF wf{};
     int nt_a = 256, nt_d = 256;
+    int cap_a = 0, cap_d = 0;
     int verbose = 0;
 };
 
@@ -183,8 +185,8 @@ FcgOut fcg_solve(const Geom &g, int q, AF af, AL al, PF pf, PL pl, LinkD df, Lin
 {
     const size_t n3 = (size_t)3 * g.Vh;
     const uint32_t nbv = fcg_nblk(n3), nbl = pt_layer_nblk(g.Vh);
-    const OpA<AF, AL, float> A{g, q, af, al, cfg.nt_a};
-    const OpA<LinkD, LinkD, double> D64{g, q, df, dl, cfg.nt_d};
+    const OpA<AF, AL, float> A{g, q, af, al, cfg.nt_a, cfg.cap_a};
+    const OpA<LinkD, LinkD, double> D64{g, q, df, dl, cfg.nt_d, cfg.cap_d};
     const size_t fb = sizeof(float2) * n3, db = sizeof(double2) * n3;
     DevBuf vr(fb), vz(cfg.learned ? fb : 0), vp(fb), vq(fb), vt(fb), vx(fb);
     // hidden channels between the two network layers as bfloat16 (48 B per site)
@@ -541,6 +543,8 @@ RefineResult fcg_refine_device(int slot, int parity, double sigma, const double 
     cfg.verbose = o.verbose;
     cfg.nt_a = tuned_nt(slot, amode);
     cfg.nt_d = tuned_nt(slot, KQ_MODE_FP64);
+    cfg.cap_a = tuned_cap(slot, amode, 1, cfg.nt_a);
+    cfg.cap_d = tuned_cap(slot, KQ_MODE_FP64, 1, cfg.nt_d);
     if (use_net)
         cfg.wf = pt_pack(learned().w, slot_norms(slot));
     const LinkD df = link_d(c.fat[slot]), dl = link_d(c.lng[slot]);

@@ -121,7 +121,7 @@ static KuQCDStatus apply(int nrhs, const void *src, void *dst, int parity, int m
 
 // ---------------------------------------------------------------- timing
 template <class LF, class LL, typename VR>
-static double time_dslash(LF fat, LL lng, const Geom &g, int nrhs, int nr, int nt, int iters)
+static double time_dslash(LF fat, LL lng, const Geom &g, int nrhs, int nr, int nt, int iters, int cap)
 {
     Context &c = ctx();
     const size_t n3 = (size_t)3 * g.Vh;
@@ -136,14 +136,14 @@ static double time_dslash(LF fat, LL lng, const Geom &g, int nrhs, int nr, int n
     a.src = Vec<VR>{s.as<c2_t<VR>>(), g.Vh, n3};
     a.dst = Vec<VR>{d.as<c2_t<VR>>(), g.Vh, n3};
     for (int i = 0; i < 2; ++i)
-        launch_dslash(a, EPI_STORE, nr, nt, c.stream);
+        launch_dslash(a, EPI_STORE, nr, nt, c.stream, cap);
     hipEvent_t e0, e1;
     KQ_HIP_CHECK(hipEventCreate(&e0));
     KQ_HIP_CHECK(hipEventCreate(&e1));
     KQ_HIP_CHECK(hipEventRecord(e0, c.stream));
     for (int i = 0; i < iters; ++i) {
         a.dst_parity = i & 1;
-        launch_dslash(a, EPI_STORE, nr, nt, c.stream);
+        launch_dslash(a, EPI_STORE, nr, nt, c.stream, cap);
     }
     KQ_HIP_CHECK(hipEventRecord(e1, c.stream));
     KQ_HIP_CHECK(hipEventSynchronize(e1));
@@ -175,7 +175,7 @@ static double site_bytes(int mode, bool dbl_gauge, bool long12, int nrhs)
 }
 
 // dispatch a timing run of the format `mode` on slot
-static double time_mode(int slot, int mode, int nrhs, int nr, int nt, int iters, bool *long12)
+static double time_mode(int slot, int mode, int nrhs, int nr, int nt, int iters, bool *long12, int cap = 0)
 {
     Context &c = ctx();
     const Geom g = slot_geom(slot);
@@ -183,12 +183,13 @@ static double time_mode(int slot, int mode, int nrhs, int nr, int nt, int iters,
     *long12 = false;
     if (mode == KQ_MODE_REFERENCE || mode == KQ_MODE_FP64) {
         if (dbl)
-            return time_dslash<LinkD, LinkD, double>(link_d(c.fat[slot]), link_d(c.lng[slot]), g, nrhs, nr, nt, iters);
-        return time_dslash<LinkF, LinkF, float>(link_f(c.fat[slot]), link_f(c.lng[slot]), g, nrhs, nr, nt, iters);
+            return time_dslash<LinkD, LinkD, double>(link_d(c.fat[slot]), link_d(c.lng[slot]), g, nrhs, nr, nt, iters,
+                                                     cap);
+        return time_dslash<LinkF, LinkF, float>(link_f(c.fat[slot]), link_f(c.lng[slot]), g, nrhs, nr, nt, iters, cap);
     }
     double ms = 0.0;
     *long12 = with_mixed_links(slot, mode, [&](auto fat, auto lng) {
-        ms = time_dslash<decltype(fat), decltype(lng), float>(fat, lng, g, nrhs, nr, nt, iters);
+        ms = time_dslash<decltype(fat), decltype(lng), float>(fat, lng, g, nrhs, nr, nt, iters, cap);
     });
     return ms;
 }
@@ -207,8 +208,10 @@ std::pair<int, int> tuned_launch(int slot, int mode, int nrhs)
     // for nrhs systems is priced as ceil(nrhs / NR) such groups (the groups are independent
     // workgroups of the same kernel). Blockings go up to the smallest power of two >= nrhs:
     // the kernel pads a partial block with the last live right-hand side and writes nothing
-    // for it. FP64 stops at NR = 2 (register pressure).
-    static const int kNr[4] = {1, 2, 4, 8}, kNt[3] = {64, 128, 256};
+    // for it. FP64 stops at NR = 2 (register pressure). Each (NR, block size) is timed
+    // without an occupancy cap and with caps of 5..2 waves per SIMD; the best cap is kept
+    // (tuned_cap).
+    static const int kNr[4] = {1, 2, 4, 8}, kNt[3] = {64, 128, 256}, kCap[5] = {0, 5, 4, 3, 2};
     auto &tg = c.tune_group[{mode, g.Vh}];
     if (tg[0] == 0.0)
         tg.fill(-1.0);
@@ -221,8 +224,17 @@ std::pair<int, int> tuned_launch(int slot, int mode, int nrhs)
             continue;
         for (int j = 0; j < 3; ++j) {
             double &ms = tg[3 * i + j];
-            if (ms < 0)
-                ms = time_mode(slot, mode, nr, nr, kNt[j], 8, &l12);
+            if (ms < 0) {
+                int best_cap = 0;
+                for (int cap : kCap) {
+                    const double t = time_mode(slot, mode, nr, nr, kNt[j], 8, &l12, cap);
+                    if (ms < 0 || t < ms) {
+                        ms = t;
+                        best_cap = cap;
+                    }
+                }
+                c.tune_cap[std::make_tuple(mode, g.Vh, nr, kNt[j])] = best_cap;
+            }
             const double cost = ms * ((nrhs + nr - 1) / nr);
             if (cost < best) {
                 best = cost;
@@ -232,8 +244,24 @@ std::pair<int, int> tuned_launch(int slot, int mode, int nrhs)
     }
     c.tune[key] = pick;
     if (c.verbosity > 1)
-        log_info("autotune mode %d nrhs %d: nr %d nt %d (%.3f ms)\n", mode, nrhs, pick.first, pick.second, best);
+        log_info("autotune mode %d nrhs %d: nr %d nt %d cap %d (%.3f ms)\n", mode, nrhs, pick.first, pick.second,
+                 tuned_cap(slot, mode, pick.first, pick.second), best);
     return pick;
+}
+
+int tuned_cap(int slot, int mode, int nr, int nt)
+{
+    Context &c = ctx();
+    const Geom g = slot_geom(slot);
+    if (mode == KQ_MODE_REFERENCE && c.fat[slot].prec == KUQCD_DOUBLE)
+        mode = KQ_MODE_FP64;
+    const auto key = std::make_tuple(mode, g.Vh, nr, nt);
+    auto it = c.tune_cap.find(key);
+    if (it == c.tune_cap.end()) {
+        (void)tuned_launch(slot, mode, nr);  // times every blocking up to nr
+        it = c.tune_cap.find(key);
+    }
+    return it == c.tune_cap.end() ? 0 : it->second;
 }
 
 }  // namespace kq
@@ -297,13 +325,15 @@ KuQCDStatus kq_bench_dslash(int mode, int nrhs, int nr, int nt, int iters, KqBen
             set_error("nrhs and iters must be positive");
             return KUQCD_ERR_INVALID_PARAM;
         }
+        int cap = 0;
         if (nr <= 0 || nt <= 0) {
             const auto p = tuned_launch(slot, mode, nrhs);
             nr = nr > 0 ? nr : p.first;
             nt = nt > 0 ? nt : p.second;
+            cap = tuned_cap(slot, mode, nr, nt);
         }
         bool l12 = false;
-        const double ms = time_mode(slot, mode, nrhs, nr, nt, iters, &l12);
+        const double ms = time_mode(slot, mode, nrhs, nr, nt, iters, &l12, cap);
         const Geom g = slot_geom(slot);
         const bool dbl = ctx().fat[slot].prec == KUQCD_DOUBLE;
         out->ms = ms;

@@ -38,6 +38,20 @@ static void print_iters(int iter, int max_iter)
         log_info("CGM: # iterations %d\n", iter);
 }
 
+// occupancy caps of the hopping term for every blocking the runner may use, and the
+// FP64 term of the reliable updates (block size and caps)
+static void set_caps(RunnerTuning &t, int slot, int mode, bool reliable)
+{
+    const bool dbl = mode == KQ_MODE_FP64 || (mode == KQ_MODE_REFERENCE && ctx().fat[slot].prec == KUQCD_DOUBLE);
+    for (int i = 0; i < 4; ++i)
+        t.cap[i] = (dbl && i > 1) ? 0 : tuned_cap(slot, mode, 1 << i, t.nt);
+    if (reliable) {
+        t.nt64 = tuned_launch(slot, KQ_MODE_FP64, 1).second;
+        t.cap64[0] = tuned_cap(slot, KQ_MODE_FP64, 1, t.nt64);
+        t.cap64[1] = tuned_cap(slot, KQ_MODE_FP64, 2, t.nt64);
+    }
+}
+
 // ---------------------------------------------------------------- reference method
 template <typename T, class LK>
 static KuQCDStatus solve_ref_t(int slot, LK fat, LK lng, void **x, void *b, KuQCDInvertParam &p)
@@ -49,6 +63,7 @@ static KuQCDStatus solve_ref_t(int slot, LK fat, LK lng, void **x, void *b, KuQC
     RunnerTuning tune;
     tune.nr = 1;
     tune.nt = tuned_launch(slot, KQ_MODE_REFERENCE, 1).second;
+    set_caps(tune, slot, KQ_MODE_REFERENCE, false);
     tune.chunk = 16;
     CgmRunner<T, T, LK, LK> run(g, parity, fat, lng, 1, ns, c.stream, tune);
     DevBuf stage, bdev(sizeof(c2_t<T>) * 3 * (size_t)g.Vh);
@@ -229,6 +244,7 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
         tune.nr = P.o->nr;
     else
         tune.nr_by_live = nr_table(P.slot, P.o->mode, max_sys);
+    set_caps(tune, P.slot, P.o->mode, true);
     x_out.alloc(sizeof(double2) * nrhs * ns * n3);
 
     auto t0 = clk::now();
@@ -366,6 +382,7 @@ static void run_fp64(Problem &P, DevBuf &x_out, KqSolveReport *rep, std::vector<
         tune.nr = P.o->nr;
     else
         tune.nr_by_live = nr_table(P.slot, KQ_MODE_FP64, P.nrhs);
+    set_caps(tune, P.slot, KQ_MODE_FP64, false);
     tune.chunk = P.o->chunk > 0 ? P.o->chunk : 16;
     auto t0 = clk::now();
     CgmRunner<double, double, LinkD, LinkD> run(P.g, P.parity, link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.nrhs,

@@ -40,5 +40,7 @@ template <typename VR, typename HR> inline void download_vec(Vec<VR> v, int r, v
 
 // autotuned (nr, nt) for the hopping term
 std::pair<int, int> tuned_launch(int slot, int mode, int nrhs);
+// occupancy cap (waves per SIMD, 0 = none) measured best for (nr, nt) by tuned_launch
+int tuned_cap(int slot, int mode, int nr, int nt);
 
 }  // namespace kq

@@ -199,8 +199,12 @@ struct RunnerTuning {
     int nr = 1;      // right-hand sides per thread in the batched hopping term
     int nt = 256;    // threads per block of the hopping term
     int chunk = 8;   // iterations replayed between host checks
+    int cap[4] = {0, 0, 0, 0};  // occupancy cap of the hopping term for nr = 1, 2, 4, 8
+    int nt64 = 256;             // FP64 hopping term of the reliable updates: block size
+    int cap64[2] = {0, 0};      // and occupancy cap for nr = 1, 2
     // nr by number of live systems (index clamped to the last entry); empty: always nr
     std::vector<int> nr_by_live;
+    int cap_for(int r) const { return cap[r >= 8 ? 3 : r >= 4 ? 2 : r >= 2 ? 1 : 0]; }
     int pick_nr(int live) const
     {
         if (nr_by_live.empty())
@@ -506,7 +510,8 @@ private:
     // ax64 = A_0 x_hi[.][0] for the n listed systems (FP64 links, read once for up to 4 systems)
     void apply_a64_x0(int n)
     {
-        const int nr = n >= 4 ? 4 : n >= 2 ? 2 : 1;
+        const int nr = n >= 2 ? 2 : 1;
+        const int cap = tune_.cap64[nr - 1];
         DslashArgs<LinkD, LinkD, double> a{};
         a.g = g_;
         a.fat = fat64_;
@@ -520,14 +525,14 @@ private:
         a.dst_parity = 1 - parity_;
         a.src = x0;
         a.dst = t;
-        launch_dslash(a, EPI_STORE, nr, 256, st_);
+        launch_dslash(a, EPI_STORE, nr, tune_.nt64, st_, cap);
         a.dst_parity = parity_;
         a.src = t;
         a.dst = ax;
         a.aux = x0;
         a.sigma = sigma_.as<double>();
         a.partial = part_a_.as<double>();
-        launch_dslash(a, EPI_AXMY_DOT, nr, 256, st_);
+        launch_dslash(a, EPI_AXMY_DOT, nr, tune_.nt64, st_, cap);
     }
 
     void start_pending()
@@ -604,14 +609,14 @@ private:
         d.dst_parity = 1 - parity_;
         d.src = vec_p0();
         d.dst = vec(tmp_);
-        launch_dslash(d, EPI_STORE, nr, tune_.nt, st_);
+        launch_dslash(d, EPI_STORE, nr, tune_.nt, st_, tune_.cap_for(nr));
         d.dst_parity = parity_;
         d.src = vec(tmp_);
         d.dst = vec(s_);
         d.aux = vec_p0();
         d.sigma = sigma_.as<double>();
         d.partial = part_a_.as<double>();
-        launch_dslash(d, EPI_AXMY_DOT, nr, tune_.nt, st_);
+        launch_dslash(d, EPI_AXMY_DOT, nr, tune_.nt, st_, tune_.cap_for(nr));
         cgm_scalars1<S>(st, L, st_);
         cgm_axpy_norm<S, VR>(st, vec(r_), vec(s_), L, st_);
         cgm_scalars2<S>(st, L, st_);
