@@ -408,21 +408,26 @@ __global__ void __launch_bounds__(kBlasNT) k_rel_fold(const CgmState<S> *st, dou
         return;
     const int m = st[rh].maxTermFold;
     const bool keep = st[rh].discard == 0;
+    // streaming accesses except the store of the unshifted solution, which the FP64
+    // hopping term of the reliable update reads next
     for (int k = 0; k < m; ++k) {
         const size_t e = off[rh] + (size_t)k * n3 + i;
         V2 *xl = reinterpret_cast<V2 *>(x_lo) + e;
         double2 *xh = reinterpret_cast<double2 *>(x_hi) + e;
         if (keep) {
-            const V2 l = *xl;
-            double2 h = *xh;
+            const V2 l = ld_nt(xl);
+            double2 h = ld_nt(xh);
             h.x += (double)l.x;
             h.y += (double)l.y;
-            *xh = h;
+            if (k == 0)
+                *xh = h;
+            else
+                st_nt(xh, h);
         }
         V2 z;
         z.x = 0;
         z.y = 0;
-        *xl = z;
+        st_nt(xl, z);
     }
 }
 

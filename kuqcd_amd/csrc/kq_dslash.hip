@@ -57,7 +57,10 @@ __device__ __forceinline__ void hop_bwd(const A &a, const LK &lk, uint32_t s, co
 }
 
 // output store: streaming for the FP64 and FP32 link formats (measured 5% and 4%
-// faster at 32^4), plain for the 16-bit one (3% slower with streaming stores)
+// faster at 32^4), plain for the 16-bit one (3% slower with streaming stores). Also
+// streaming where the next hopping term gathers the output (MdaggM's intermediate):
+// plain stores there made the 32^4 solves 1-2% slower. The choice must stay
+// compile-time: a run-time flag in this store made the FP64 term almost 2x slower.
 template <class LF, class V2> __device__ __forceinline__ void put(V2 *p, V2 v)
 {
     if constexpr (std::is_same<LF, LinkH>::value)
