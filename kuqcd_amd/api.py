@@ -103,6 +103,11 @@ def kuqcd_solve_multishift(x: list, b: np.ndarray | None, param: InvertParam | N
     return L.lib().kq_solve_multishift(xp, _ptr(b), C.byref(param.struct()) if param is not None else None)
 
 
+def kq_synchronize() -> int:
+    """Wait for all work the library queued on its stream (0 on success)."""
+    return L.lib().kq_synchronize()
+
+
 def kuqcd_gauge_free() -> None:
     L.lib().kq_gauge_free()
 

@@ -3,10 +3,12 @@
 // field, each with its own list of shifts (a multishift system) or a single
 // shift (a refinement system started from an initial guess).
 //
-// The iteration is captured once in a hipGraph and replayed in chunks; all
-// recurrence scalars live in device memory (CgmState), so a chunk needs no host
-// round trip, and one batched hopping-term launch per half-iteration serves
-// every live system, so each link is read from HBM once for all of them.
+// One iteration is captured in a hipGraph (one per right-hand-side blocking and
+// group count, with every grid sized for the systems that can be live) and
+// replayed in chunks; all recurrence scalars live in device memory (CgmState),
+// so a chunk needs no host round trip, and one batched hopping-term launch per
+// half-iteration serves every live system, so each link is read from HBM once
+// for all of them.
 // Between chunks the host reads the states back, and in mixed precision
 // performs reliable updates: the FP32 iterated solutions are folded into FP64
 // accumulators and the base residual is recomputed in FP64, r = b - A_0 x,

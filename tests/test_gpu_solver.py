@@ -100,6 +100,33 @@ def test_cgm_modes_reach_tolerance(gpu, mode):
 
 
 @pytest.mark.gpu
+def test_repeated_solves_reuse_workspace(gpu):
+    """Solves of different shapes in a row (cached device buffers are reused by size, the
+    autotuner runs again after each gauge reload): every solve reaches the tolerance and agrees
+    with the first solve of its shape (the autotuner may pick another block size, which
+    regroups the reduction partial sums, so agreement is to rounding, not bitwise)."""
+    kq = gpu
+    results = {}
+    for rnd in range(2):
+        for lat in [(8, 8, 8, 8), (4, 4, 4, 8), (8, 8, 8, 8)]:
+            fat, lng, b, shifts, sig_abs, tol = _config1(kq, lat)
+            kq.load_gauge_pair(fat, lng, lat)
+            for mode in (1, 4):
+                x = [np.zeros_like(b) for _ in shifts]
+                prm = kq.InvertParam(shifts=shifts, tol=tol, max_iter=5000)
+                st, rep = kq.solve_ex(x, b, prm, kq.default_options(mode=mode, refine=1, verbose=0))
+                assert st == kq.KUQCD_SUCCESS, kq.last_error()
+                assert max(rep.true_rel_res[k] for k in range(len(shifts))) <= 1e-10
+                key = (lat, mode)
+                if key in results:
+                    assert max(rel_err(a, c) for a, c in zip(x, results[key])) <= 1e-9
+                else:
+                    results[key] = x
+            kq.kuqcd_gauge_free()  # also releases the cached device memory
+    assert kq.kq_synchronize() == 0
+
+
+@pytest.mark.gpu
 def test_cgm_batched_matches_single(gpu):
     kq = gpu
     lat = (8, 4, 4, 8)
