@@ -50,7 +50,7 @@ typedef enum {
 
 typedef struct {
     int mode;              /* KqMode */
-    double reliable_delta; /* reliable update when |r| drops below delta * |r| at the last one (0.1) */
+    double reliable_delta; /* reliable update when |r| drops below delta * |r| at the last one (0.02) */
     int refine;            /* 1: FP64 true-residual check of every shift + batched refinement */
     double rel_tol;        /* per-shift target of ||b - A_k x_k|| / ||b||; 0: sqrt(tol)/||b|| */
     int learned;           /* 1: learned-preconditioner flexible CG for the lightest shift */
