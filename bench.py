@@ -199,6 +199,12 @@ def main() -> int:
         t = time.perf_counter()
         _, it_done = oracle.cgm(fat, lng, lat, 0, shifts, bo, tol, n_it)
         t_it = (time.perf_counter() - t) / it_done
+        model = "unknown CPU"
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+        except OSError:
+            pass
         cpu = {
             "value": round(t_it * rep64.iter, 3),
             "unit": "s",
@@ -206,7 +212,7 @@ def main() -> int:
             "kind": "port",
             "sample": (f"oracle C restatement of CGM::Invert, FP64, same 32^4 12-shift problem, first {it_done} "
                        f"iterations timed ({t_it * 1e3:.1f} ms/iteration with all 12 shifts active) x the "
-                       f"{rep64.iter} iterations of the FP64 solve"),
+                       f"{rep64.iter} iterations of the FP64 solve; {nthreads} OpenMP threads on {model}"),
         }
         log(f"cpu: {t_it * 1e3:.1f} ms/iter x {rep64.iter} = {cpu['value']:.1f}s ({nthreads} threads)")
 
