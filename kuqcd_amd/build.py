@@ -23,7 +23,10 @@ OBJDIR = PKG / "_build"
 LIB = LIBDIR / "libkuqcd.so"
 
 SOURCES = [
-    "kq_dslash.hip",
+    "kq_dslash_fp64.hip",
+    "kq_dslash_fp32.hip",
+    "kq_dslash_mixed.hip",
+    "kq_dslash_16.hip",
     "kq_cgm_kernels.hip",
     "kq_layout.hip",
     "kq_ptnet.hip",

@@ -8,6 +8,11 @@
 // neighbour tables are read. Follows DiracHISQ::Dslash/FwdDslash/BwdDslash
 // (dirac_hisq.cpp:409-502) with the 1/2 and c3000 factors folded into the
 // device links.
+//
+// Implementation header: kq_dslash_*.hip instantiate launch_dslash for the link
+// formats (one translation unit per family, so they compile in parallel).
+#pragma once
+
 #include "kq_dslash.h"
 
 #include <algorithm>
@@ -238,12 +243,5 @@ void launch_dslash(DslashArgs<LF, LL, SR> a, int epi, int nr, int nt, hipStream_
     }
 }
 
-template void launch_dslash<LinkD, LinkD, double>(DslashArgs<LinkD, LinkD, double>, int, int, int, hipStream_t, int);
-template void launch_dslash<LinkF, LinkF, float>(DslashArgs<LinkF, LinkF, float>, int, int, int, hipStream_t, int);
-template void launch_dslash<LinkF, LinkF12, float>(DslashArgs<LinkF, LinkF12, float>, int, int, int, hipStream_t, int);
-template void launch_dslash<LinkH, LinkH, float>(DslashArgs<LinkH, LinkH, float>, int, int, int, hipStream_t, int);
-template void launch_dslash<LinkF, LinkH, float>(DslashArgs<LinkF, LinkH, float>, int, int, int, hipStream_t, int);
-template void launch_dslash<LinkF, LinkH12, float>(DslashArgs<LinkF, LinkH12, float>, int, int, int, hipStream_t, int);
-template void launch_dslash<LinkH, LinkH12, float>(DslashArgs<LinkH, LinkH12, float>, int, int, int, hipStream_t, int);
 
 }  // namespace kq
