@@ -110,7 +110,11 @@ static std::vector<double> true_res2(int slot, int parity, const double *b, cons
     const int nl = (int)which.size();
     DevBuf tmp(sizeof(double2) * nk * n3), ax(sizeof(double2) * nk * n3), sig(sizeof(double) * nk);
     DevBuf list(sizeof(int) * (nl + 1));
-    const int nt = 256, nr = nl >= 2 ? 2 : 1;
+    // FP64 blocking, block size and occupancy cap from the autotuner (one group of at most
+    // two right-hand sides per workgroup)
+    const auto pick = tuned_launch(slot, KQ_MODE_FP64, nl);
+    const int nr = std::min(pick.first, 2), nt = pick.second;
+    const int cap = tuned_cap(slot, KQ_MODE_FP64, nr, nt);
     const uint32_t nblk_d = dslash_nblk(g.Vh, nt);
     DevBuf part(sizeof(double) * nk * std::max(nblk_d, blas_nblk(g.Vh)));
     std::vector<int> hl(which);
@@ -130,14 +134,14 @@ static std::vector<double> true_res2(int slot, int parity, const double *b, cons
     a.dst_parity = 1 - parity;
     a.src = xv;
     a.dst = tv;
-    launch_dslash(a, EPI_STORE, nr, nt, c.stream);
+    launch_dslash(a, EPI_STORE, nr, nt, c.stream, cap);
     a.dst_parity = parity;
     a.src = tv;
     a.dst = av;
     a.aux = xv;
     a.sigma = sig.as<double>();
     a.partial = part.as<double>();
-    launch_dslash(a, EPI_AXMY_DOT, nr, nt, c.stream);
+    launch_dslash(a, EPI_AXMY_DOT, nr, nt, c.stream, cap);
     const uint32_t nb = blas_nblk(g.Vh);
     resid2_partial(b, ax.as<double>(), nl, list.as<int>(), g.Vh, part.as<double>(), nb, c.stream);
     const std::vector<double> all = reduce_host(part.as<double>(), nb, nk, c.stream);
