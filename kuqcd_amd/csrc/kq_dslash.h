@@ -27,6 +27,9 @@ template <class LF, class LL, typename SR> struct DslashArgs {
     uint32_t nblk;        // grid.x, the partial-sum stride
 };
 
+// block sizes launch_dslash accepts: 64, 128 and 256 threads
+constexpr int kMinDslashNT = 64;
+
 // Launch D over all sites of parity dst_parity for every right-hand side.
 // nr = right-hand sides handled per thread (1, 2, 4 or 8), nt = threads per block,
 // cap = at most this many wavefronts per SIMD (0: as many as the registers allow). The

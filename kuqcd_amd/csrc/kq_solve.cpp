@@ -47,6 +47,13 @@ static void set_caps(RunnerTuning &t, int slot, int mode, bool reliable)
         t.cap[i] = (dbl && i > 1) ? 0 : tuned_cap(slot, mode, 1 << i, t.nt);
     if (reliable) {
         t.nt64 = tuned_launch(slot, KQ_MODE_FP64, 1).second;
+        // KQ_RELIABLE_NT=64|128|256 fixes the FP64 term's block size (to test the runner
+        // with block sizes that differ between its two hopping terms)
+        if (const char *e = getenv("KQ_RELIABLE_NT")) {
+            const int v = atoi(e);
+            if (v == 64 || v == 128 || v == 256)
+                t.nt64 = v;
+        }
         t.cap64[0] = tuned_cap(slot, KQ_MODE_FP64, 1, t.nt64);
         t.cap64[1] = tuned_cap(slot, KQ_MODE_FP64, 2, t.nt64);
     }

@@ -243,7 +243,11 @@ public:
             state_h_[i].stop = 1;  // unused slots never run
         nblk_ = blas_nblk(g.Vh);
         nblk_dsl_ = dslash_nblk(g.Vh, tune.nt);
-        part_a_.alloc(sizeof(double) * max_sys * std::max(nblk_dsl_, dslash_nblk(g.Vh, 256)));
+        // partials of both hopping terms the runner launches: the iteration's (block size
+        // tune.nt, stride nblk_dsl_) and the FP64 one of the reliable updates (tune.nt64),
+        // whose block sizes are tuned independently; rows for the smallest block size
+        // launch_dslash accepts fit either
+        part_a_.alloc(sizeof(double) * max_sys * dslash_nblk(g.Vh, kMinDslashNT));
         part_b_.alloc(sizeof(double) * max_sys * nblk_);
         ints_.alloc(sizeof(int) * (2 * max_sys + 4));
         sigma_.alloc(sizeof(double) * max_sys);

@@ -100,6 +100,32 @@ def test_cgm_modes_reach_tolerance(gpu, mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode,nt,nt64", [(2, 256, 64), (4, 256, 64), (3, 128, 64), (4, 64, 256)])
+def test_mixed_block_sizes_of_the_two_hopping_terms(gpu, monkeypatch, mode, nt, nt64):
+    """The iteration's hopping term and the FP64 one of the reliable updates are tuned
+    independently and write their per-system partial sums into one buffer of the runner, so a
+    smaller FP64 block size (more partials per system) must still fit every system's row.
+    Forced apart here, with four shifts forking into separate systems; every shift still
+    reaches the tolerance and agrees with the FP64 oracle."""
+    kq = gpu
+    lat = (8, 8, 8, 8)
+    fat, lng, b, shifts, sig_abs, tol = _config1(kq, lat)
+    kq.load_gauge_pair(fat, lng, lat)
+    monkeypatch.setenv("KQ_RELIABLE_NT", str(nt64))
+    x = [np.zeros_like(b) for _ in shifts]
+    prm = kq.InvertParam(shifts=shifts, tol=tol, max_iter=5000)
+    print("mode", mode, "nt", nt, "nt64", nt64, flush=True)
+    st, rep = kq.solve_ex(x, b, prm, kq.default_options(mode=mode, nt=nt, refine=1, verbose=0))
+    assert st == kq.KUQCD_SUCCESS, kq.last_error()
+    assert rep.reliable_updates > 0
+    xo, _ = oracle.cgm(fat, lng, lat, 0, shifts, b, tol, 5000)
+    for k in range(len(shifts)):
+        assert _true_rel_res(fat, lng, lat, 0, b, x[k], sig_abs[k]) <= 1e-10, k
+        assert rel_err(x[k], xo[k]) <= 1e-7, k
+    assert kq.kq_synchronize() == 0
+
+
+@pytest.mark.gpu
 def test_repeated_solves_reuse_workspace(gpu):
     """Solves of different shapes in a row (cached device buffers are reused by size, the
     autotuner runs again after each gauge reload): every solve reaches the tolerance and agrees
