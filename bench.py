@@ -7,13 +7,15 @@ phases folded in, Gaussian even-site source, mass m = 0.01, 12 shifts
 log-spaced in [1e-4, 1] (absolute shifts m^2 + s_k), FP64 true relative
 residual <= 1e-10 for every shift.
 
-One step = one complete mixed-precision multishift solve (by default FP32 fat
-links, 16-bit fixed-point long links and FP32 vectors in the iteration; FP64
-reliable updates, recurrence scalars, per-shift FP64 verification and
-refinement) with the source and the solutions resident in HBM. The line also
-carries the plain FP64 GPU multishift CG time on the same problem, the
-all-FP32-links mixed time, the FP64 hopping-term roofline, and the oracle CGM
-timed on the host cores.
+One step = one complete mixed-precision multishift solve as the 32^4 workload
+specifies it: FP32 links and vectors with FP32 accumulation in the iteration,
+FP64 reliable updates whenever |r| has dropped to 10 % of its value at the last
+one (delta 0.1), FP64 recurrence scalars, per-shift FP64 verification and
+refinement, with the source and the solutions resident in HBM. The line also
+carries the plain FP64 GPU multishift CG time on the same problem, the tuned
+variants (library default delta 0.02; FP32 fat with 16-bit fixed-point long
+links), the FP64 hopping-term roofline, and the oracle CGM timed on the host
+cores.
 """
 from __future__ import annotations
 
@@ -59,8 +61,10 @@ def main() -> int:
     ap.add_argument("--nshift", type=int, default=12)
     ap.add_argument("--mass", type=float, default=0.01)
     ap.add_argument("--rel-tol", type=float, default=1e-10)
-    ap.add_argument("--mode", type=int, default=4,
+    ap.add_argument("--mode", type=int, default=2,
                     help="mixed mode: 2 = FP32 links, 3 = 16-bit links, 4 = FP32 fat + 16-bit long links")
+    ap.add_argument("--delta", type=float, default=0.1,
+                    help="reliable update when |r| falls below delta times |r| at the last one")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -118,9 +122,11 @@ def main() -> int:
     import ctypes as C
     xp = (C.c_void_p * args.nshift)(*x_dev)
 
-    def solve(mode):
+    def solve(mode, delta=None):
         prm = A.InvertParam(shifts=shifts, tol=tol, max_iter=40000)
         opt = A.default_options(mode=mode, refine=1, rel_tol=args.rel_tol, verbose=0)
+        if delta is not None:
+            opt.reliable_delta = delta
         rep = A.L.KqSolveReport()
         st = L.kq_solve_multishift_ex(xp, C.c_void_p(b_dev), C.byref(prm.struct()), C.byref(opt), C.byref(rep))
         if st != A.KUQCD_SUCCESS:
@@ -128,13 +134,13 @@ def main() -> int:
         return rep
 
     for w in range(args.warmup):
-        rep = solve(args.mode)
+        rep = solve(args.mode, args.delta)
         log(f"warmup {w}: {rep.seconds:.3f}s iter {rep.iter}")
     assert L.kq_synchronize() == 0
     t0 = time.perf_counter()
     reps = []
     for k in range(args.steps):
-        reps.append(solve(args.mode))
+        reps.append(solve(args.mode, args.delta))
         log(f"step {k}: {reps[-1].seconds:.3f}s iter {reps[-1].iter} refine {list(reps[-1].refine_iter[:args.nshift])}")
     assert L.kq_synchronize() == 0
     t1 = time.perf_counter()
@@ -152,15 +158,20 @@ def main() -> int:
 
     x_mixed = download()
 
-    # the all-FP32-links mixed mode, for comparison
-    sec_fp32 = None
-    if args.mode != 2:
-        solve(2)  # warm (autotune / derived formats)
-        t = time.perf_counter()
-        rep32 = solve(2)
+    # tuned variants: the library's default reliable-update interval (delta 0.02), and FP32 fat
+    # links with 16-bit fixed-point long links (same FP64 verification of every shift)
+    variants = {}
+    for key, mode, delta in (("fp32_links_delta_0.02", 2, None), ("fp32_fat_16bit_long_delta_0.02", 4, None)):
+        solve(mode, delta)  # warm (autotune / derived formats)
         assert L.kq_synchronize() == 0
-        sec_fp32 = time.perf_counter() - t
-        log(f"mixed with FP32 links: {sec_fp32:.3f}s iter {rep32.iter}")
+        t = time.perf_counter()
+        for _ in range(2):
+            rv = solve(mode, delta)
+        assert L.kq_synchronize() == 0
+        sv = (time.perf_counter() - t) / 2
+        variants[key] = {"seconds": round(sv, 4), "iterations": rv.iter, "reliable_updates": rv.reliable_updates,
+                         "max_true_rel_res": max(rv.true_rel_res[k] for k in range(args.nshift))}
+        log(f"variant {key}: {sv:.3f}s iter {rv.iter}")
 
     # plain FP64 GPU multishift CG on the same problem
     solve(1)  # warm (autotune / derived formats)
@@ -232,10 +243,10 @@ def main() -> int:
         "config": {"workload": f"HISQ even-odd multishift CG, {'x'.join(map(str, lat))}, {args.nshift} shifts "
                                f"log-spaced 1e-4..1, m={args.mass}, FP64 true rel. residual {args.rel_tol:g} per shift",
                    "lattice": list(lat), "nshift": args.nshift, "mass": args.mass,
-                   "mode": MODE_NAME[args.mode], "parallelism": "single GPU"},
+                   "mode": MODE_NAME[args.mode], "reliable_delta": args.delta, "parallelism": "single GPU"},
         "fp64_seconds": round(sec_fp64, 4),
         "speedup_vs_fp64": round(sec_fp64 / sec_mixed, 3),
-        "mixed_fp32_links_seconds": round(sec_fp32, 4) if sec_fp32 else None,
+        "variants": {k: dict(v, speedup_vs_fp64=round(sec_fp64 / v["seconds"], 3)) for k, v in variants.items()},
         "iterations": {"mixed_multishift": rep.iter, "mixed_reliable_updates": rep.reliable_updates,
                        "mixed_refine": [rep.refine_iter[k] for k in range(args.nshift)], "fp64": rep64.iter},
         "max_true_rel_res": {"mixed": max(res_mixed), "fp64": max(res_fp64)},
