@@ -94,25 +94,60 @@ __device__ __forceinline__ void tap_comb(const Geom &g, int qo, const LK &lk, ui
         v[c] = mk<float2>(acc[2 * c], acc[2 * c + 1]);
 }
 
-// the same with the channels as bfloat16 (16 B per colour: 4 channels' (re, im) pairs)
-template <class LK, int MU, int H, bool NEG>
-__device__ __forceinline__ void tap_comb16(const Geom &g, int qo, const LK &lk, uint32_t s, const SiteCoord &sc,
-                                           const kq_v4u *in, const float2 wt[kPtHidden], float2 v[3])
+// Layer 2 with bfloat16 channels, the four taps of direction MU (t = 4 MU + 0..3: hops +1,
+// +3, -1, -3, as KQ_PT_FOR_TAPS orders them) added to acc in tap order. Every gather and link
+// load of the four is issued before any of them is used: written tap by tap, the compiler
+// waited for each 16-byte gather before issuing the next (81 of 102 loads followed by a full
+// wait) and the kernel ran at a third of the HBM rate.
+template <class LF, class LL, int MU>
+__device__ __forceinline__ void taps16_dir(const Geom &g, int qo, const LF &fat, const LL &lng, uint32_t s,
+                                           const SiteCoord &sc, const kq_v4u *in, const PtWeightsBF &w,
+                                           float2 acc[3])
 {
-    const uint32_t n = nb_index<MU>(g, s, sc, NEG ? -H : H);
-    float2 x[3];
+    const uint32_t n[4] = {nb_index<MU>(g, s, sc, 1), nb_index<MU>(g, s, sc, 3), nb_index<MU>(g, s, sc, -1),
+                           nb_index<MU>(g, s, sc, -3)};
+    kq_v4u u[4][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const kq_v4u u = in[(size_t)c * g.Vh + n];
-        const float2 p0 = cmulf(wt[0], unpack_w(u.x)), p1 = cmulf(wt[1], unpack_w(u.y));
-        const float2 p2 = cmulf(wt[2], unpack_w(u.z)), p3 = cmulf(wt[3], unpack_w(u.w));
-        x[c] = mk<float2>((p0.x + p1.x) + (p2.x + p3.x), (p0.y + p1.y) + (p2.y + p3.y));
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            u[t][c] = in[(size_t)c * g.Vh + n[t]];
+    float2 U[4][9];
+    fat.load(qo, MU, s, U[0]);
+    lng.load(qo, MU, s, U[1]);
+    fat.load(1 - qo, MU, n[2], U[2]);
+    lng.load(1 - qo, MU, n[3], U[3]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        float2 wt[kPtHidden];
+#pragma unroll
+        for (int c = 0; c < kPtHidden; ++c)
+            wt[c] = unpack_w(w.w2[c][4 * MU + t]);
+        float2 x[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float2 p0 = cmulf(wt[0], unpack_w(u[t][c].x)), p1 = cmulf(wt[1], unpack_w(u[t][c].y));
+            const float2 p2 = cmulf(wt[2], unpack_w(u[t][c].z)), p3 = cmulf(wt[3], unpack_w(u[t][c].w));
+            x[c] = mk<float2>((p0.x + p1.x) + (p2.x + p3.x), (p0.y + p1.y) + (p2.y + p3.y));
+        }
+        float v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            v[k] = 0;
+        if (t < 2) {
+            umv_add(U[t], x, v);
+        } else {
+            udmv_sub(U[t], x, v);
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                v[k] = -v[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            acc[k].x += v[2 * k];
+            acc[k].y += v[2 * k + 1];
+        }
     }
-    float acc[6];
-    tap_mul<LK, NEG>(lk, qo, MU, s, n, x, acc);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-        v[c] = mk<float2>(acc[2 * c], acc[2 * c + 1]);
 }
 
 // the 16 taps as (t, mu, hop, negative)
@@ -195,7 +230,8 @@ __global__ void __launch_bounds__(kNT) k_pt_layer1(const PtLayerArgs a, LF fat, 
 
 // layer-2 form: out(x) = w0 skip(x) + sum_t T_t( sum_c w2[c][t] in_c )(x); H16: channels from in16
 template <class LF, class LL, bool H16>
-__global__ void __launch_bounds__(kNT) k_pt_layer2(const PtLayerArgs a, LF fat, LL lng)
+__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(1, 4)))
+k_pt_layer2(const PtLayerArgs a, LF fat, LL lng)
 {
     if (a.stop && *a.stop)
         return;
@@ -207,16 +243,17 @@ __global__ void __launch_bounds__(kNT) k_pt_layer2(const PtLayerArgs a, LF fat, 
 #pragma unroll
     for (int k = 0; k < 3; ++k)
         acc[k] = a.skip ? cmulf(w0, a.skip[(size_t)k * a.g.Vh + s]) : mk<float2>(0.f, 0.f);
+    if constexpr (H16) {
+        taps16_dir<LF, LL, 0>(a.g, a.qo, fat, lng, s, sc, a.in16, a.w, acc);
+        taps16_dir<LF, LL, 1>(a.g, a.qo, fat, lng, s, sc, a.in16, a.w, acc);
+        taps16_dir<LF, LL, 2>(a.g, a.qo, fat, lng, s, sc, a.in16, a.w, acc);
+        taps16_dir<LF, LL, 3>(a.g, a.qo, fat, lng, s, sc, a.in16, a.w, acc);
+    } else {
 #define KQ_TAP2(T, MU, H, NEG)                                                     \
     {                                                                              \
         float2 wt[kPtHidden], v[3];                                                \
         _Pragma("unroll") for (int c = 0; c < kPtHidden; ++c) wt[c] = unpack_w(a.w.w2[c][T]); \
-        if constexpr (H16) {                                                       \
-            if (H == 1)                                                            \
-                tap_comb16<LF, MU, H, NEG>(a.g, a.qo, fat, s, sc, a.in16, wt, v);  \
-            else                                                                   \
-                tap_comb16<LL, MU, H, NEG>(a.g, a.qo, lng, s, sc, a.in16, wt, v);  \
-        } else if (H == 1)                                                         \
+        if (H == 1)                                                                \
             tap_comb<LF, MU, H, NEG>(a.g, a.qo, fat, s, sc, a.in, wt, v);          \
         else                                                                       \
             tap_comb<LL, MU, H, NEG>(a.g, a.qo, lng, s, sc, a.in, wt, v);          \
@@ -226,8 +263,9 @@ __global__ void __launch_bounds__(kNT) k_pt_layer2(const PtLayerArgs a, LF fat, 
             acc[k].y += v[k].y;                                                    \
         }                                                                          \
     }
-    KQ_PT_FOR_TAPS(KQ_TAP2)
+        KQ_PT_FOR_TAPS(KQ_TAP2)
 #undef KQ_TAP2
+    }
     if (valid) {
 #pragma unroll
         for (int k = 0; k < 3; ++k)
