@@ -37,7 +37,9 @@ def _rows(path):
 
 
 def _short(name):
-    n = name.split("(")[0].replace("void ", "").replace("kq::", "")
+    # drop the namespaces first: "(anonymous namespace)::" would otherwise end the name at its "("
+    n = name.replace("(anonymous namespace)::", "").replace("void ", "").replace("kq::", "")
+    n = n.split("(")[0]
     return n if len(n) <= 70 else n[:67] + "..."
 
 
