@@ -12,6 +12,9 @@ hold at any size (the oracle's CGM is too slow to solve these, its FP64 operator
   * four sources at 32^4, each with its own 12 shifts: one batched solve reaches 1e-10 on all
     48 shift solutions (reported; three shifts per source recomputed by the oracle) and
     agrees with the four single solves.
+  * the FP64 hopping term against the oracle's at 32^4 (both parities), 24^3x48, 30^4 and
+    48^3x96 (<= 1e-13 relative, as at the small sizes): the tiled XCD site order with power-of-two
+    and other planes, and the contiguous order it falls back to when Lz % 4 or Vh % 2048 is not 0.
   * learned preconditioner at 24^4, m = 0.005: trained on solver pairs of 16 gauge fields, the
     flexible CG refines a held-out source from 1e-6 to 1e-10 in at most 3/4 of plain CG's
     iterations (measured 88 against 158), without falling back.
@@ -45,6 +48,21 @@ def _solve(kq, b, shifts, mode):
     st, rep = kq.solve_ex(x, b, prm, kq.default_options(mode=mode, refine=1, rel_tol=1e-10, verbose=0))
     assert st == kq.KUQCD_SUCCESS, kq.last_error()
     return x, rep
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lat,parity", [((32, 32, 32, 32), 0), ((32, 32, 32, 32), 1), ((24, 24, 24, 48), 0),
+                                        ((30, 30, 30, 30), 1), (LAT48, 0)])
+def test_dslash_fp64_matches_oracle(gpu, lat, parity):
+    kq = gpu
+    print("hopping term", lat, "parity", parity, flush=True)
+    fat, lng = kq.gen_gauge(lat, kind=kq.KQ_GEN_WEAK, eps=0.3, fat_scale=0.9, seed_fat=31, seed_long=32)
+    kq.load_gauge_pair(fat, lng, lat)
+    src = kq.gen_source(lat, 1 - parity, seed=33)
+    dst = np.empty_like(src)
+    assert kq.dslash(src, dst, parity) == 0, kq.last_error()
+    ref = oracle.dslash(fat, lng, src, lat, parity)
+    assert rel_err(dst, ref) <= 1e-13
 
 
 @pytest.fixture(scope="module")
