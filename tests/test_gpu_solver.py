@@ -2,9 +2,10 @@
 and with the reference's own known-answer test (test/test_solver.cpp).
 
 Tolerances:
-  * reference method, FP64: same iteration count as the oracle (+-1, the
-    reductions add in a different order) and solutions within 1e-8 relative
-    (both stop at an iterated relative residual of 1e-10).
+  * reference method, FP64: the oracle's iteration count within 1 % (at least 2:
+    the reductions add in a different order, whose grouping also follows the
+    autotuned block size) and solutions within 1e-8 relative (both stop at an
+    iterated relative residual of 1e-10).
   * every FP64 / mixed solve: recomputed FP64 true relative residual of every
     shift <= the requested 1e-10.
 """
@@ -70,7 +71,7 @@ def test_cgm_fp64_matches_oracle_config1(gpu):
     assert kq.kuqcd_solve_multishift(x, b, prm) == kq.KUQCD_SUCCESS, kq.last_error()
     xo, it = oracle.cgm(fat, lng, lat, 0, shifts, b, tol, 5000)
     print("iterations gpu", prm.iter, "oracle", it)
-    assert abs(prm.iter - it) <= 1
+    assert abs(prm.iter - it) <= max(2, it // 100)
     for k in range(len(shifts)):
         assert rel_err(x[k], xo[k]) <= 1e-8, k
         assert _true_rel_res(fat, lng, lat, 0, b, x[k], sig_abs[k]) <= 1.5e-10
