@@ -50,7 +50,8 @@ typedef enum {
 
 typedef struct {
     int mode;              /* KqMode */
-    double reliable_delta; /* reliable update when |r| drops below delta * |r| at the last one (0.02) */
+    double reliable_delta; /* reliable update when |r| drops below delta * |r| at the last one;
+                              0 (default): 0.02, or 0.1 with 16-bit fat links (KQ_MODE_MIXED_FP16) */
     int refine;            /* 1: FP64 true-residual check of every shift + batched refinement */
     double rel_tol;        /* per-shift target of ||b - A_k x_k|| / ||b||; 0: sqrt(tol)/||b|| */
     int learned;           /* 1: learned-preconditioner flexible CG for the lightest shift */

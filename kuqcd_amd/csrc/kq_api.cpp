@@ -479,7 +479,7 @@ void kq_default_options(KqSolveOptions *opt)
     if (!opt)
         return;
     opt->mode = KQ_MODE_MIXED_FP32;
-    opt->reliable_delta = 0.02;
+    opt->reliable_delta = 0.0;  // the mode's default
     opt->refine = 1;
     opt->rel_tol = 0.0;
     opt->learned = 0;
