@@ -246,6 +246,10 @@ KQ_API int kq_device_count(void);
 /* host check of the fast division used by the kernels' index math: returns
  * the number of n in [0, n_max) with n / d != the fast quotient */
 KQ_API long kq_selftest_fastdiv(uint32_t d, uint32_t n_max);
+/* host check of the reliable-update fold rule of the mixed solver: 1 if the fold adds the
+ * low-precision update of shift k to its FP64 solution, for a system with max_term shifts still
+ * iterating and the host's divergence flag `discard` */
+KQ_API int kq_selftest_fold_keeps(int discard, int k, int max_term);
 /* checkerboard index of the neighbour of site s (parity q) at hop h along mu,
  * computed by the kernels' index math on the host */
 KQ_API long kq_neighbour_index(const int lat[4], int q, long s, int mu, int h);

@@ -81,7 +81,16 @@ void copy_words(const void *src, void *dst, size_t bytes, hipStream_t stream);
 
 // list[0..n) = src[0..n), *count = n (src may be mapped pinned host memory)
 void set_int_list(const int *src, int *list, int *count, int n, hipStream_t stream);
-// x_hi[k] += x_lo[k]; x_lo[k] = 0 for k < st[sys].maxTermFold
+// Whether the fold adds x_lo[k] to x_hi[k]. A system whose iteration diverged
+// (discard set by the host) drops the unfolded updates of the shifts that are still
+// iterating (k < maxTerm); the shifts that converged and were dropped since the last
+// fold (maxTerm <= k < maxTermFold) finished before the divergence was seen, and keep
+// their updates.
+__host__ __device__ inline bool fold_keeps(int discard, int k, int max_term)
+{
+    return discard == 0 || k >= max_term;
+}
+// x_hi[k] += x_lo[k] (if fold_keeps); x_lo[k] = 0 for k < st[sys].maxTermFold
 template <typename S, typename VR>
 void rel_fold(CgmState<S> *st, double *x_hi, VR *x_lo, const size_t *off, uint32_t Vh, const int *list, const int *n,
               int nsys, hipStream_t stream);

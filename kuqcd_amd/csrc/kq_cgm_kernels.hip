@@ -407,14 +407,14 @@ __global__ void __launch_bounds__(kBlasNT) k_rel_fold(const CgmState<S> *st, dou
     if (i >= n3)
         return;
     const int m = st[rh].maxTermFold;
-    const bool keep = st[rh].discard == 0;
+    const int discard = st[rh].discard, live = st[rh].maxTerm;
     // streaming accesses except the store of the unshifted solution, which the FP64
     // hopping term of the reliable update reads next
     for (int k = 0; k < m; ++k) {
         const size_t e = off[rh] + (size_t)k * n3 + i;
         V2 *xl = reinterpret_cast<V2 *>(x_lo) + e;
         double2 *xh = reinterpret_cast<double2 *>(x_hi) + e;
-        if (keep) {
+        if (fold_keeps(discard, k, live)) {
             const V2 l = ld_nt(xl);
             double2 h = ld_nt(xh);
             h.x += (double)l.x;

@@ -29,6 +29,12 @@ template <class LF, class LL, typename SR> struct DslashArgs {
 
 // block sizes launch_dslash accepts: 64, 128 and 256 threads
 constexpr int kMinDslashNT = 64;
+// Block size of the hopping terms whose partial sums feed the reference method, the plain
+// FP64 solve and the FP64 verification. One partial per workgroup, added in a fixed order,
+// makes a reduction deterministic for a given block size; fixing the block size (instead of
+// taking the autotuner's timing-dependent pick) makes these results bitwise reproducible from
+// one process to the next.
+constexpr int kReproNT = 256;
 
 // Launch D over all sites of parity dst_parity for every right-hand side.
 // nr = right-hand sides handled per thread (1, 2, 4 or 8), nt = threads per block,

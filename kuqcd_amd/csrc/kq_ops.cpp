@@ -438,6 +438,11 @@ long kq_selftest_fastdiv(uint32_t d, uint32_t n_max)
     return bad;
 }
 
+int kq_selftest_fold_keeps(int discard, int k, int max_term)
+{
+    return fold_keeps(discard, k, max_term) ? 1 : 0;
+}
+
 long kq_neighbour_index(const int lat[4], int q, long s, int mu, int h)
 {
     const Geom g = make_geom(lat);

@@ -69,7 +69,7 @@ static KuQCDStatus solve_ref_t(int slot, LK fat, LK lng, void **x, void *b, KuQC
     const int ns = p.num_shift;
     RunnerTuning tune;
     tune.nr = 1;
-    tune.nt = tuned_launch(slot, KQ_MODE_REFERENCE, 1).second;
+    tune.nt = kReproNT;
     set_caps(tune, slot, KQ_MODE_REFERENCE, false);
     tune.chunk = 16;
     CgmRunner<T, T, LK, LK> run(g, parity, fat, lng, 1, ns, c.stream, tune);
@@ -117,10 +117,10 @@ static std::vector<double> true_res2(int slot, int parity, const double *b, cons
     const int nl = (int)which.size();
     DevBuf tmp(sizeof(double2) * nk * n3), ax(sizeof(double2) * nk * n3), sig(sizeof(double) * nk);
     DevBuf list(sizeof(int) * (nl + 1));
-    // FP64 blocking, block size and occupancy cap from the autotuner (one group of at most
-    // two right-hand sides per workgroup)
+    // FP64 blocking and occupancy cap from the autotuner (one group of at most two
+    // right-hand sides per workgroup); the fixed block size keeps the residuals reproducible
     const auto pick = tuned_launch(slot, KQ_MODE_FP64, nl);
-    const int nr = std::min(pick.first, 2), nt = pick.second;
+    const int nr = std::min(pick.first, 2), nt = kReproNT;
     const int cap = tuned_cap(slot, KQ_MODE_FP64, nr, nt);
     const uint32_t nblk_d = dslash_nblk(g.Vh, nt);
     DevBuf part(sizeof(double) * nk * std::max(nblk_d, blas_nblk(g.Vh)));
@@ -388,7 +388,7 @@ static void run_fp64(Problem &P, DevBuf &x_out, KqSolveReport *rep, std::vector<
     Context &c = ctx();
     const size_t n3 = (size_t)3 * P.g.Vh;
     RunnerTuning tune;
-    tune.nt = P.o->nt > 0 ? P.o->nt : tuned_launch(P.slot, KQ_MODE_FP64, 1).second;
+    tune.nt = P.o->nt > 0 ? P.o->nt : kReproNT;
     if (P.o->nr > 0)
         tune.nr = P.o->nr;
     else

@@ -56,6 +56,18 @@ def test_fastdiv_exact(kq, d):
     assert kq.L.lib().kq_selftest_fastdiv(d, 1 << 21) == 0
 
 
+def test_fold_rule_after_divergence(kq):
+    """The mixed solver's fold of the low-precision solution updates into the FP64 solutions.
+    Normally every shift k < maxTermFold is folded. When the host has flagged the iteration as
+    diverged (discard), only the shifts still iterating (k < maxTerm) lose their updates; shifts
+    dropped as converged earlier in the same chunk (maxTerm <= k < maxTermFold) keep theirs."""
+    f = kq.L.lib().kq_selftest_fold_keeps
+    for max_term in range(0, 5):
+        for k in range(0, 6):
+            assert f(0, k, max_term) == 1
+            assert f(1, k, max_term) == (1 if k >= max_term else 0), (k, max_term)
+
+
 @pytest.mark.parametrize("lat", [(4, 4, 4, 1), (6, 4, 2, 8), (2, 1, 3, 5), (8, 6, 4, 2), (4, 6, 1, 3)])
 def test_neighbour_index_matches_oracle(kq, lat):
     Vh = int(np.prod(lat)) // 2
