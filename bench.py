@@ -12,10 +12,12 @@ specifies it: FP32 links and vectors with FP32 accumulation in the iteration,
 FP64 reliable updates whenever |r| has dropped to 10 % of its value at the last
 one (delta 0.1), FP64 recurrence scalars, per-shift FP64 verification and
 refinement, with the source and the solutions resident in HBM. The line also
-carries the plain FP64 GPU multishift CG time on the same problem, the tuned
-variants (library default delta 0.02; FP32 fat with 16-bit fixed-point long
-links), the FP64 hopping-term roofline, and the oracle CGM timed on the host
-cores.
+carries the plain FP64 GPU multishift CG time on the same problem, the same
+headline solve with the source and solutions in host memory (PCIe transfers
+included), the tuned variants (library default delta 0.02; FP32 fat with 16-bit
+fixed-point long links), the FP64 hopping-term roofline, and the oracle CGM run
+to completion on the host cores (OMP_PROC_BIND=close, OMP_PLACES=cores), whose
+solution also gives each shift's error of the mixed and FP64 solves.
 """
 from __future__ import annotations
 
@@ -23,6 +25,7 @@ import argparse
 import json
 import os
 import re
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -52,7 +55,25 @@ def pmc_traffic(path: Path):
     return float(m.group(1)) if m else None
 
 
+def host_cpu() -> dict:
+    """CPU model, physical cores (lscpu) and the CPUs this process may run on."""
+    info = {"model": "unknown CPU", "physical_cores": None, "cpus_allowed": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict(ln.split(":", 1) for ln in out.splitlines() if ":" in ln)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name", info["model"])
+        if "Core(s) per socket" in kv and "Socket(s)" in kv:
+            info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
 def main() -> int:
+    # the oracle's OpenMP threads: one per core, packed (read when its library is loaded)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -66,7 +87,6 @@ def main() -> int:
     ap.add_argument("--delta", type=float, default=0.1,
                     help="reliable update when |r| falls below delta times |r| at the last one")
     ap.add_argument("--fp64-steps", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the hopping terms (FP64, FP32, FP16) and the copy kernel, for counter runs")
@@ -158,6 +178,28 @@ def main() -> int:
 
     x_mixed = download()
 
+    # the same headline solve with the source and the solutions in host memory (the reference
+    # boundary: H2D of b and D2H of the 12 solutions inside the timed region)
+    xh = [np.zeros_like(b) for _ in range(args.nshift)]
+    xhp = (C.c_void_p * args.nshift)(*[a.ctypes.data for a in xh])
+
+    def solve_host():
+        prm = A.InvertParam(shifts=shifts, tol=tol, max_iter=40000)
+        opt = A.default_options(mode=args.mode, refine=1, rel_tol=args.rel_tol, verbose=0)
+        opt.reliable_delta = args.delta
+        rep = A.L.KqSolveReport()
+        st = L.kq_solve_multishift_ex(xhp, C.c_void_p(b.ctypes.data), C.byref(prm.struct()), C.byref(opt),
+                                      C.byref(rep))
+        if st != A.KUQCD_SUCCESS:
+            raise RuntimeError(f"solve failed: {st} {A.last_error()}")
+
+    t = time.perf_counter()
+    for _ in range(2):
+        solve_host()
+    sec_host = (time.perf_counter() - t) / 2
+    log(f"host-buffer solve: {sec_host:.3f}s")
+    del xh
+
     # tuned variants: the library's default reliable-update interval (delta 0.02), and FP32 fat
     # links with 16-bit fixed-point long links (same FP64 verification of every shift)
     variants = {}
@@ -183,7 +225,8 @@ def main() -> int:
     sec_fp64 = (t3 - t2) / args.fp64_steps
     res_fp64 = [rep64.true_rel_res[k] for k in range(args.nshift)]
     # per-shift solution difference, mixed vs the FP64 solve
-    diff = [float(np.linalg.norm(xm - x6) / np.linalg.norm(x6)) for xm, x6 in zip(x_mixed, download())]
+    x64 = download()
+    diff = [float(np.linalg.norm(xm - x6) / np.linalg.norm(x6)) for xm, x6 in zip(x_mixed, x64)]
     log(f"fp64: {sec_fp64:.3f}s iter {rep64.iter}; mixed {sec_mixed:.3f}s")
 
     # hopping-term roofline (HIP events on the library stream)
@@ -202,35 +245,37 @@ def main() -> int:
     r64 = rl["fp64"]
     traffic = pmc_traffic(ROOT / "profiles" / "dslash_32x32x32x32_pmc.txt")
 
-    # CPU baseline: the oracle CGM (same algorithm) on the host cores, bounded sample
+    # CPU baseline: the oracle CGM (the reference's algorithm, FP64) solving the same problem to
+    # completion on the host cores; its solution also gives each shift's error
     cpu = None
+    err_vs_oracle = None
     if not args.no_cpu:
         import oracle
         nthreads = oracle.num_threads()
+        hc = host_cpu()
         bo = np.ascontiguousarray(b)
         t = time.perf_counter()
-        oracle.cgm(fat, lng, lat, 0, shifts, bo, tol, 3)
-        t_cal = (time.perf_counter() - t) / 3
-        n_it = int(max(3, min(2000, args.cpu_seconds / max(t_cal, 1e-6))))
-        t = time.perf_counter()
-        _, it_done = oracle.cgm(fat, lng, lat, 0, shifts, bo, tol, n_it)
-        t_it = (time.perf_counter() - t) / it_done
-        model = "unknown CPU"
-        try:
-            with open("/proc/cpuinfo") as f:
-                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
-        except OSError:
-            pass
+        xo, it_o = oracle.cgm(fat, lng, lat, 0, shifts, bo, tol, 40000)
+        t_cpu = time.perf_counter() - t
+        err_vs_oracle = {
+            "mixed": [float(f"{np.linalg.norm(a - o) / np.linalg.norm(o):.2e}") for a, o in zip(x_mixed, xo)],
+            "fp64": [float(f"{np.linalg.norm(a - o) / np.linalg.norm(o):.2e}") for a, o in zip(x64, xo)],
+            "oracle_iterations": it_o,
+        }
         cpu = {
-            "value": round(t_it * rep64.iter, 3),
+            "value": round(t_cpu, 3),
             "unit": "s",
             "cores": nthreads,
             "kind": "port",
-            "sample": (f"oracle C restatement of CGM::Invert, FP64, same 32^4 12-shift problem, first {it_done} "
-                       f"iterations timed ({t_it * 1e3:.1f} ms/iteration with all 12 shifts active) x the "
-                       f"{rep64.iter} iterations of the FP64 solve; {nthreads} OpenMP threads on {model}"),
+            "sample": (f"the whole solve: oracle C restatement of CGM::Invert (FP64, converged shifts dropped as "
+                       f"inverter_cgm.cpp:71-76 does), same 32^4 12-shift problem and stop test, {it_o} iterations, "
+                       f"{t_cpu / it_o * 1e3:.1f} ms/iteration; {nthreads} OpenMP threads (OMP_PROC_BIND="
+                       f"{os.environ.get('OMP_PROC_BIND')}, OMP_PLACES={os.environ.get('OMP_PLACES')}) on "
+                       f"{hc['cpus_allowed']} allowed CPUs of a {hc['model']} host with {hc['physical_cores']} "
+                       f"physical cores"),
         }
-        log(f"cpu: {t_it * 1e3:.1f} ms/iter x {rep64.iter} = {cpu['value']:.1f}s ({nthreads} threads)")
+        log(f"cpu: {t_cpu:.1f}s, {it_o} iterations ({nthreads} threads); worst mixed error against the oracle "
+            f"{max(err_vs_oracle['mixed']):.2e}")
 
     line = {
         "metric": METRIC,
@@ -248,7 +293,9 @@ def main() -> int:
         "config": {"workload": f"HISQ even-odd multishift CG, {'x'.join(map(str, lat))}, {args.nshift} shifts "
                                f"log-spaced 1e-4..1, m={args.mass}, FP64 true rel. residual {args.rel_tol:g} per shift",
                    "lattice": list(lat), "nshift": args.nshift, "mass": args.mass,
-                   "mode": MODE_NAME[args.mode], "reliable_delta": args.delta, "parallelism": "single GPU"},
+                   "mode": MODE_NAME[args.mode], "reliable_delta": args.delta, "parallelism": "single GPU",
+                   "b_x": "device-resident (reference host layout in HBM); host-buffer time in seconds_host_buffers"},
+        "seconds_host_buffers": round(sec_host, 4),
         "fp64_seconds": round(sec_fp64, 4),
         "speedup_vs_fp64": round(sec_fp64 / sec_mixed, 3),
         "variants": {k: dict(v, speedup_vs_fp64=round(sec_fp64 / v["seconds"], 3)) for k, v in variants.items()},
@@ -256,6 +303,7 @@ def main() -> int:
                        "mixed_refine": [rep.refine_iter[k] for k in range(args.nshift)], "fp64": rep64.iter},
         "max_true_rel_res": {"mixed": max(res_mixed), "fp64": max(res_fp64)},
         "rel_diff_mixed_vs_fp64": [float(f"{d:.2e}") for d in diff],
+        "rel_err_vs_oracle": err_vs_oracle,
         "roofline": {"bound": "hbm", "achieved": round(r64.gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r64.gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_dslash<LinkD,LinkD,double> (FP64 hopping term, 2400 B/site)",

@@ -34,3 +34,12 @@ def test_traffic_summary_parses():
 def test_profiles_state_their_command(name):
     text = (ROOT / "profiles" / name).read_text()
     assert "made by" in text and "rocprofv3" in text and "bench.py" in text
+
+
+def test_host_cpu_description():
+    import bench
+
+    hc = bench.host_cpu()
+    assert hc["cpus_allowed"] >= 1
+    assert hc["physical_cores"] is None or hc["physical_cores"] >= 1
+    assert isinstance(hc["model"], str) and hc["model"]

@@ -1,17 +1,22 @@
-"""The solver's measured configurations at their full sizes, checked through properties that
-hold at any size (the oracle's CGM is too slow to solve these, its FP64 operator is not):
+"""The solver's measured configurations at their full sizes:
 
   * 32^4, m = 0.01, 12 shifts log-spaced 1e-4..1, FP64 true relative residual 1e-10 on every
-    shift: the oracle's FP64 operator recomputes each shift's residual (<= 1e-10, and the
-    library's own report agrees to 1e-3 relative), for the FP64 and the mixed solve; the two
-    solutions agree shift by shift (asserted at 1e-8; measured <= 1.5e-9).
+    shift. The oracle's CGM solves the problem once in FP64 (343 iterations, ~20 s on 16 host
+    threads). The drop-in FP64 solve (kuqcd_solve_multishift) matches its iteration count
+    within 1 % (at least 2) and every shift's solution within 1e-8. Every mixed mode (FP32
+    links at the reliable-update interval of the workload, delta 0.1, and at the library
+    default; FP32 fat with 16-bit long links) reaches 1e-10 on every shift, recomputed by the
+    oracle's FP64 operator (the library's own report agrees to 1e-3 relative), and each
+    shift's error against the oracle solution is printed and asserted <= 1e-8 (measured
+    <= 1.5e-9 against the GPU FP64 solve).
   * 48^3x96, m = 0.005, 12 shifts 1e-5..1, 16-bit links with 12-real long links, and FP32 fat
-    with 16-bit long links: every shift's reported FP64 residual <= 1e-10, and the oracle's
+    with 16-bit long links: every shift's reported FP64 residual <= 1e-10, the oracle's
     recomputation of the lightest, a middle and the heaviest shift agrees (one application of
-    the oracle's operator takes seconds at this size).
-  * four sources at 32^4, each with its own 12 shifts: one batched solve reaches 1e-10 on all
-    48 shift solutions (reported; three shifts per source recomputed by the oracle) and
-    agrees with the four single solves.
+    the oracle's operator takes seconds at this size, its CGM would take minutes), and each
+    shift's error against the GPU's FP64 solve is printed (asserted <= 1e-7).
+  * N = 4, 8 and 16 sources at 32^4, each with its own 12 shifts drawn log-uniform in
+    [1e-4, 1]: one batched solve reaches 1e-10 on all N x 12 shift solutions (reported; three
+    shifts per source recomputed by the oracle) and agrees with the N single solves.
   * the FP64 hopping term against the oracle's at 32^4 (both parities), 24^3x48, 30^4 and
     48^3x96 (<= 1e-13 relative, as at the small sizes): the tiled XCD site order with power-of-two
     and other planes, and the contiguous order it falls back to when Lz % 4 or Vh % 2048 is not 0.
@@ -41,11 +46,14 @@ def _log_shifts(mass, lo, ns):
     return np.concatenate([[mass * mass + s[0]], s[1:] - s[0]]), mass * mass + s
 
 
-def _solve(kq, b, shifts, mode):
+def _solve(kq, b, shifts, mode, delta=0.0):
     x = [np.zeros_like(b) for _ in shifts]
     prm = kq.InvertParam(shifts=shifts, tol=0.0, max_iter=40000)
-    print(f"solve: {b.shape[0]} sites, {len(shifts)} shifts, mode {mode}", flush=True)
-    st, rep = kq.solve_ex(x, b, prm, kq.default_options(mode=mode, refine=1, rel_tol=1e-10, verbose=0))
+    print(f"solve: {b.shape[0]} sites, {len(shifts)} shifts, mode {mode}, delta {delta}", flush=True)
+    opt = kq.default_options(mode=mode, refine=1, rel_tol=1e-10, verbose=0)
+    if delta > 0:
+        opt.reliable_delta = delta
+    st, rep = kq.solve_ex(x, b, prm, opt)
     assert st == kq.KUQCD_SUCCESS, kq.last_error()
     return x, rep
 
@@ -71,33 +79,63 @@ def gauge32(kq):
     return fat, lng
 
 
-@pytest.mark.gpu
-def test_32x32x32x32_12_shifts_fp64_and_mixed(gpu, gauge32):
-    kq = gpu
+@pytest.fixture(scope="module")
+def oracle32(gauge32, kq):
+    """The oracle's FP64 CGM on the 32^4 workload (reference shift and tolerance conventions)."""
     fat, lng = gauge32
-    kq.load_gauge_pair(fat, lng, LAT32)
     b = kq.gen_source(LAT32, 0, seed=3)
     shifts, sig = _log_shifts(0.01, 1e-4, 12)
-    sols = {}
-    for mode in (kq.KQ_MODE_FP64, kq.KQ_MODE_MIXED_FP32_L16):
-        x, rep = _solve(kq, b, shifts, mode)
-        for k in range(len(shifts)):
-            r = _true_rel_res(fat, lng, LAT32, 0, b, x[k], sig[k])
-            assert r <= 1e-10, (mode, k, r)
-            assert abs(r - rep.true_rel_res[k]) <= 1e-3 * r + 1e-15, (mode, k, r, rep.true_rel_res[k])
-        sols[mode] = x
-    d = [rel_err(sols[kq.KQ_MODE_MIXED_FP32_L16][k], sols[kq.KQ_MODE_FP64][k]) for k in range(len(shifts))]
-    print(f"mixed vs FP64, worst shift: {max(d):.2e}", flush=True)
+    tol = 1e-20 * float(np.sum(b * b))
+    print("oracle CGM at 32^4 ...", flush=True)
+    xo, it = oracle.cgm(fat, lng, LAT32, 0, shifts, b, tol, 40000)
+    print(f"oracle CGM: {it} iterations", flush=True)
+    return b, shifts, sig, tol, xo, it
+
+
+@pytest.mark.gpu
+def test_32x32x32x32_fp64_drop_in_matches_oracle_cgm(gpu, gauge32, oracle32):
+    kq = gpu
+    fat, lng = gauge32
+    b, shifts, sig, tol, xo, ito = oracle32
+    kq.load_gauge_pair(fat, lng, LAT32)
+    x = [np.zeros_like(b) for _ in shifts]
+    prm = kq.InvertParam(shifts=shifts, tol=tol, max_iter=40000)
+    assert kq.kuqcd_solve_multishift(x, b, prm) == kq.KUQCD_SUCCESS, kq.last_error()
+    d = [rel_err(x[k], xo[k]) for k in range(len(shifts))]
+    print(f"FP64: {prm.iter} iterations (oracle {ito}); worst shift against the oracle {max(d):.2e}", flush=True)
+    assert abs(prm.iter - ito) <= max(2, ito // 100), (prm.iter, ito)
     assert max(d) <= 1e-8, d
 
 
 @pytest.mark.gpu
-def test_32x32x32x32_four_sources_batched_matches_single(gpu, gauge32):
+def test_32x32x32x32_12_shifts_fp64_and_mixed(gpu, gauge32, oracle32):
+    kq = gpu
+    fat, lng = gauge32
+    b, shifts, sig, _, xo, _ = oracle32
+    kq.load_gauge_pair(fat, lng, LAT32)
+    runs = [("FP64", kq.KQ_MODE_FP64, 0.0), ("FP32 links, delta 0.1", kq.KQ_MODE_MIXED_FP32, 0.1),
+            ("FP32 links, default delta", kq.KQ_MODE_MIXED_FP32, 0.0),
+            ("FP32 fat + 16-bit long links", kq.KQ_MODE_MIXED_FP32_L16, 0.0)]
+    for name, mode, delta in runs:
+        x, rep = _solve(kq, b, shifts, mode, delta)
+        res = [_true_rel_res(fat, lng, LAT32, 0, b, x[k], sig[k]) for k in range(len(shifts))]
+        err = [rel_err(x[k], xo[k]) for k in range(len(shifts))]
+        print(f"{name}: {rep.iter} iterations; worst residual {max(res):.2e}; error against the oracle "
+              f"per shift {' '.join(f'{e:.1e}' for e in err)}", flush=True)
+        for k in range(len(shifts)):
+            assert res[k] <= 1e-10, (name, k, res[k])
+            assert abs(res[k] - rep.true_rel_res[k]) <= 1e-3 * res[k] + 1e-15, (name, k, res[k], rep.true_rel_res[k])
+        assert max(err) <= 1e-8, (name, err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nrhs,mode", [(4, 4), (8, 2), (16, 2)])
+def test_32x32x32x32_sources_batched_matches_single(gpu, gauge32, nrhs, mode):
     kq = gpu
     fat, lng = gauge32
     kq.load_gauge_pair(fat, lng, LAT32)
-    nrhs, ns, mass = 4, 12, 0.01
-    rng = np.random.default_rng(4)
+    ns, mass = 12, 0.01
+    rng = np.random.default_rng(4 + nrhs)
     bs = [kq.gen_source(LAT32, 0, seed=100 + r) for r in range(nrhs)]
     sh = np.zeros((nrhs, ns))
     sig = np.zeros((nrhs, ns))
@@ -105,9 +143,8 @@ def test_32x32x32x32_four_sources_batched_matches_single(gpu, gauge32):
         s = np.sort(10 ** rng.uniform(-4, 0, ns))
         sh[r] = np.concatenate([[mass * mass + s[0]], s[1:] - s[0]])
         sig[r] = mass * mass + s
-    mode = kq.KQ_MODE_MIXED_FP32_L16
     xs = [np.zeros_like(bs[0]) for _ in range(nrhs * ns)]
-    print("batched solve: 4 sources x 12 shifts", flush=True)
+    print(f"batched solve: {nrhs} sources x 12 shifts, mode {mode}", flush=True)
     st, reps = kq.solve_batched(xs, bs, kq.InvertParam(shifts=sh, tol=0.0, max_iter=40000),
                              kq.default_options(mode=mode, refine=1, rel_tol=1e-10, verbose=0))
     assert st == kq.KUQCD_SUCCESS, kq.last_error()
@@ -129,14 +166,22 @@ def test_48x48x48x96_12_shifts_16bit_links(gpu, kq):
     kq.load_gauge_pair(fat, lng, LAT48)
     b = kq.gen_source(LAT48, 0, seed=7)
     shifts, sig = _log_shifts(0.005, 1e-5, 12)
+    x64, rep64 = _solve(kq, b, shifts, kq.KQ_MODE_FP64)
+    print(f"FP64: {rep64.iter} iterations, {rep64.seconds_multishift:.2f}s multishift phase", flush=True)
+    assert max(rep64.true_rel_res[:len(shifts)]) <= 1e-10
     for mode in (kq.KQ_MODE_MIXED_FP16, kq.KQ_MODE_MIXED_FP32_L16):
         x, rep = _solve(kq, b, shifts, mode)
-        print(f"mode {mode}: {rep.iter} iterations, {rep.seconds_multishift:.2f}s multishift phase", flush=True)
+        d = [rel_err(x[k], x64[k]) for k in range(len(shifts))]
+        print(f"mode {mode}: {rep.iter} iterations, {rep.seconds_multishift:.2f}s multishift phase; error "
+              f"against the FP64 solve per shift {' '.join(f'{e:.1e}' for e in d)}", flush=True)
         assert max(rep.true_rel_res[:len(shifts)]) <= 1e-10, mode
         for k in (0, len(shifts) // 2, len(shifts) - 1):
             r = _true_rel_res(fat, lng, LAT48, 0, b, x[k], sig[k])
             assert r <= 1e-10, (mode, k, r)
             assert abs(r - rep.true_rel_res[k]) <= 1e-3 * r + 1e-15, (mode, k)
+        # both stop at 1e-10 true residual; the lightest system's condition number (~1e5) lets
+        # their errors differ by far more than that, measured 1.1e-8
+        assert max(d) <= 1e-7, (mode, d)
         del x
 
 
