@@ -5,6 +5,10 @@
  *
  *   init -> load FAT -> load LONG -> solve* -> gauge_free -> end
  *
+ * Like the reference header (kuqcd.h:18-22) it also declares the inner classes
+ * (Lattice, LattBase, SpinorField, GaugeField, DiracHISQ, CGM and the BLAS-1
+ * functions: kuqcd_classes.h), here over fields in GPU memory.
+ *
  * The reference header also pulls in <mpi.h> and <omp.h>; this one includes
  * <mpi.h> only when the caller's toolchain has it, since the library runs one
  * process per GPU and does not call MPI itself.
@@ -14,6 +18,7 @@
 
 #include <cstdio>
 #include "kuqcd_enum.h"
+#include "kuqcd_classes.h"
 
 #if defined(__has_include)
 #if __has_include(<mpi.h>)

@@ -31,11 +31,13 @@ SOURCES = [
     "kq_layout.hip",
     "kq_ptnet.hip",
     "kq_fcg.hip",
+    "kq_blas.hip",
     "kq_api.cpp",
     "kq_solve.cpp",
     "kq_learned.cpp",
     "kq_ops.cpp",
     "kq_gen.cpp",
+    "kq_classes.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

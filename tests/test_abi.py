@@ -189,3 +189,30 @@ def test_cpp_dropin_compiles():
     r = subprocess.run(["make", "-s", "-C", str(ROOT / "tests" / "cpp")], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert out.exists()
+
+
+INNER = [("float", 1), ("double", 1), ("float", 16), ("double", 8)]
+
+
+@pytest.mark.parametrize("T,nb", INNER)
+def test_exports_inner_classes(kq, T, nb):
+    """kuqcd_classes.h's instantiations (dirac_hisq.h:60-68, spinor_field.h:76-84, inverter_cgm.h)
+    are exported with default visibility."""
+    _, dem = _exported()
+    sf = f"kuqcd::SpinorField<{T}, {nb}, 1>"
+    gf = f"kuqcd::GaugeField<{T}, {nb}, (KuQCDReconstructType)18>"
+    dh = f"kuqcd::DiracHISQ<{T}, {nb}, (KuQCDReconstructType)18, (KuQCDReconstructType)18>"
+    want = [f"{sf}::SpinorField(kuqcd::LattBase&)", f"{sf}::SetValues({T} const*)", f"{sf}::GetValues({T}*) const",
+            f"{sf}::Clear()", f"{gf}::GaugeField(kuqcd::LattBase&, kuqcd::CommBase&)", f"{gf}::SetValues({T} const*)",
+            f"{dh}::Dslash({sf} const&, {sf}&, bool)", f"{dh}::MdaggM({sf} const&, {sf}&, bool)",
+            f"kuqcd::CGM<{T}, {sf} >::Invert(int, {T}*, {sf} const&, {sf}**, double, int)",
+            f"{T} kuqcd::Norm2<{T}, {nb}, 1>({sf} const&)", f"{T} kuqcd::RealDot<{T}, {nb}, 1>({sf} const&, {sf} const&)",
+            f"void kuqcd::Axpby<{T}, {nb}, 1>({sf}&, {sf} const&, {T}, {T})"]
+    for w in want:
+        assert w in dem, w
+
+
+def test_cpp_callers_build_without_a_gpu():
+    """The C++ callers compile against include/kuqcd.h alone and link against libkuqcd.so."""
+    subprocess.run(["make", "-s", "-C", str(ROOT / "tests" / "cpp")], check=True, timeout=300)
+    assert (ROOT / "tests" / "cpp" / "inner_api_test").exists()
