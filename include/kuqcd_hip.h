@@ -103,6 +103,16 @@ KQ_API KuQCDStatus kq_dslash(const void *src, void *dst, int dst_parity, int mod
 KQ_API KuQCDStatus kq_dslash_batched(int nrhs, const void *src, void *dst, int dst_parity, int mode, int nr, int nt);
 /* dst = sigma*src - D D src on parity `parity` (the CGM operator A) */
 KQ_API KuQCDStatus kq_mdaggm(const void *src, void *dst, int parity, double sigma, int mode);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): kq_mdaggm in
+ * KQ_MODE_REFERENCE (the loaded gauge's precision) without any allocation. src and dst are
+ * device memory in the reference host layout (Vh*6 reals each, not overlapping); dst also holds
+ * the intermediate D src during the call. The library's scratch for it is one static device
+ * array of KQ_NOALLOC_SCRATCH_BYTES per GPU (the FP64 operator up to 32^4 sites); a larger
+ * lattice returns KUQCD_ERR_INVALID_PARAM. The work is queued on the library stream and not
+ * waited for (kq_synchronize waits). Calls that overlap on one GPU race for the scratch. */
+#define KQ_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+KQ_API KuQCDStatus kq_mdaggm_noalloc(const void *src, void *dst, int parity, double sigma);
 
 /* ---- timing hooks (device-resident inputs, HIP events on the library stream) */
 typedef struct {

@@ -190,6 +190,7 @@ _PROTOS = {
     "kq_dslash": (C.c_int, [_VP, _VP, C.c_int, C.c_int]),
     "kq_dslash_batched": (C.c_int, [C.c_int, _VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int]),
     "kq_mdaggm": (C.c_int, [_VP, _VP, C.c_int, C.c_double, C.c_int]),
+    "kq_mdaggm_noalloc": (C.c_int, [_VP, _VP, C.c_int, C.c_double]),
     "kq_bench_dslash": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(KqBenchResult)]),
     "kq_bench_copy": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(KqBenchResult)]),
     "kq_autotune_dslash": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),

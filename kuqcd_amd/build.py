@@ -32,6 +32,7 @@ SOURCES = [
     "kq_ptnet.hip",
     "kq_fcg.hip",
     "kq_blas.hip",
+    "kq_noalloc.hip",
     "kq_api.cpp",
     "kq_solve.cpp",
     "kq_learned.cpp",

@@ -39,6 +39,7 @@ void ensure_device()
         throw std::runtime_error("no HIP device is visible");
     KQ_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     DevPool::get().set_stream(c.stream);
+    (void)noalloc_scratch();  // the no-allocation entry point's scratch address on this device
     c.device_ready = true;
 }
 

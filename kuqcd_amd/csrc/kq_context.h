@@ -111,6 +111,10 @@ template <class Fn> bool with_mixed_links(int slot, int mode, Fn &&fn)
     return d.f32_long12;
 }
 
+// device address of the no-allocation entry point's static scratch array on the current
+// device (looked up once per device)
+void *noalloc_scratch();
+
 void log_info(const char *fmt, ...);
 void set_error(const std::string &msg);
 
