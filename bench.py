@@ -15,7 +15,8 @@ refinement, with the source and the solutions resident in HBM. The line also
 carries the plain FP64 GPU multishift CG time on the same problem, the same
 headline solve with the source and solutions in host memory (PCIe transfers
 included), the tuned variants (library default delta 0.02; FP32 fat with 16-bit
-fixed-point long links), the FP64 hopping-term roofline, and the oracle CGM run
+fixed-point long links; the headline mode with the learned-preconditioner
+refinement of shift 0), the FP64 hopping-term roofline, and the oracle CGM run
 to completion on the host cores (OMP_PROC_BIND=close, OMP_PLACES=cores), whose
 solution also gives each shift's error of the mixed and FP64 solves.
 """
@@ -74,6 +75,7 @@ def main() -> int:
     # the oracle's OpenMP threads: one per core, packed (read when its library is loaded)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
+    hc = host_cpu()  # before any OpenMP runtime binds this thread to one place
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -81,6 +83,7 @@ def main() -> int:
     ap.add_argument("--lattice", default="32,32,32,32")
     ap.add_argument("--nshift", type=int, default=12)
     ap.add_argument("--mass", type=float, default=0.01)
+    ap.add_argument("--shift-min", type=float, default=1e-4, help="smallest of the log-spaced shifts (largest 1)")
     ap.add_argument("--rel-tol", type=float, default=1e-10)
     ap.add_argument("--mode", type=int, default=2,
                     help="mixed mode: 2 = FP32 links, 3 = 16-bit links, 4 = FP32 fat + 16-bit long links")
@@ -88,6 +91,7 @@ def main() -> int:
                     help="reliable update when |r| falls below delta times |r| at the last one")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-refinement variant")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the hopping terms (FP64, FP32, FP16) and the copy kernel, for counter runs")
     args = ap.parse_args()
@@ -126,7 +130,7 @@ def main() -> int:
         A.kuqcd_gauge_free()
         return 0
 
-    s = np.logspace(-4, 0, args.nshift)
+    s = np.logspace(np.log10(args.shift_min), 0, args.nshift)
     m2 = args.mass ** 2
     shifts = np.concatenate([[m2 + s[0]], s[1:] - s[0]])  # reference convention (shifts[k>=1] relative)
     b2 = float(np.sum(b * b))
@@ -215,6 +219,49 @@ def main() -> int:
                          "max_true_rel_res": max(rv.true_rel_res[k] for k in range(args.nshift))}
         log(f"variant {key}: {sv:.3f}s iter {rv.iter}")
 
+    # learned refinement in the headline configuration: the network is trained by the library's
+    # HIP kernels on solver pairs of 8 seeded 16^4 gauges from the same generator (it is local and
+    # gauge covariant, so the weights carry over to 32^4); shift 0 leaves the multishift at 1e-6
+    # and the flexible CG finishes it; every shift is then verified in FP64 as usual
+    if not args.no_learned:
+        lat_t = (16, 16, 16, 16)
+        t = time.perf_counter()
+        A.learned_clear()
+        for sd in range(8):
+            f_t, l_t = A.gen_gauge(lat_t, kind=A.KQ_GEN_WEAK, eps=0.3, fat_scale=0.9, c_naik=-1.0 / 24.0,
+                                   seed_fat=2 * (100 + sd) + 1, seed_long=2 * (100 + sd) + 2)
+            A.load_gauge_pair(f_t, l_t, lat_t)
+            A.learned_collect(0, shifts[0], nsrc=1, seed=1000 + sd, rel_tol=1e-11, min_level=1e-8)
+        trep = A.learned_train(steps=200, batch=8, lr=3e-4, lr_final=3e-5, seed=1, init=2)
+        A.learned_clear()
+        t_train = time.perf_counter() - t
+        A.load_gauge_pair(fat, lng, lat)
+
+        def solve_learned():
+            prm = A.InvertParam(shifts=shifts, tol=tol, max_iter=40000)
+            opt = A.default_options(mode=args.mode, refine=1, rel_tol=args.rel_tol, verbose=0, learned=1)
+            opt.reliable_delta = args.delta
+            rep = A.L.KqSolveReport()
+            st = L.kq_solve_multishift_ex(xp, C.c_void_p(b_dev), C.byref(prm.struct()), C.byref(opt), C.byref(rep))
+            if st != A.KUQCD_SUCCESS:
+                raise RuntimeError(f"learned solve failed: {st} {A.last_error()}")
+            return rep
+
+        solve_learned()  # warm
+        assert L.kq_synchronize() == 0
+        t = time.perf_counter()
+        for _ in range(2):
+            rl_ = solve_learned()
+        assert L.kq_synchronize() == 0
+        sv = (time.perf_counter() - t) / 2
+        variants["learned_refinement"] = {
+            "seconds": round(sv, 4), "iterations": rl_.iter, "reliable_updates": rl_.reliable_updates,
+            "shift0_fcg_iterations": rl_.refine_iter[0], "learned_used": rl_.learned_used,
+            "max_true_rel_res": max(rl_.true_rel_res[k] for k in range(args.nshift)),
+            "training": f"8 gauges 16^4, {trep.nsamples} solver pairs, loss {trep.loss_initial:.3f} -> "
+                        f"{trep.loss_final:.3f}, {t_train:.1f}s (outside the timed solves)"}
+        log(f"variant learned: {sv:.3f}s iter {rl_.iter} shift-0 FCG {rl_.refine_iter[0]} used {rl_.learned_used}")
+
     # plain FP64 GPU multishift CG on the same problem
     solve(1)  # warm (autotune / derived formats)
     t2 = time.perf_counter()
@@ -252,7 +299,6 @@ def main() -> int:
     if not args.no_cpu:
         import oracle
         nthreads = oracle.num_threads()
-        hc = host_cpu()
         bo = np.ascontiguousarray(b)
         t = time.perf_counter()
         xo, it_o = oracle.cgm(fat, lng, lat, 0, shifts, bo, tol, 40000)
@@ -276,6 +322,16 @@ def main() -> int:
         }
         log(f"cpu: {t_cpu:.1f}s, {it_o} iterations ({nthreads} threads); worst mixed error against the oracle "
             f"{max(err_vs_oracle['mixed']):.2e}")
+        # the oracle's FP64 hopping term on the same cores: median of 20 applications
+        src_o = np.ascontiguousarray(xo[0])
+        ts = []
+        for _ in range(20):
+            t = time.perf_counter()
+            oracle.dslash(fat, lng, src_o, lat, 1)
+            ts.append(time.perf_counter() - t)
+        t_d = float(np.median(ts))
+        cpu["dslash_fp64"] = {"ms": round(t_d * 1e3, 2), "GB/s": round(2400 * Vh / t_d / 1e9, 1),
+                              "GFLOP/s": round(1146 * Vh / t_d / 1e9, 1)}
 
     line = {
         "metric": METRIC,
@@ -291,7 +347,8 @@ def main() -> int:
         "dtype": MODE_DTYPE[args.mode],
         "data": "synthetic (seeded weak-field SU(3) links, Gaussian source)",
         "config": {"workload": f"HISQ even-odd multishift CG, {'x'.join(map(str, lat))}, {args.nshift} shifts "
-                               f"log-spaced 1e-4..1, m={args.mass}, FP64 true rel. residual {args.rel_tol:g} per shift",
+                               f"log-spaced {args.shift_min:g}..1, m={args.mass}, FP64 true rel. residual "
+                               f"{args.rel_tol:g} per shift",
                    "lattice": list(lat), "nshift": args.nshift, "mass": args.mass,
                    "mode": MODE_NAME[args.mode], "reliable_delta": args.delta, "parallelism": "single GPU",
                    "b_x": "device-resident (reference host layout in HBM); host-buffer time in seconds_host_buffers"},

@@ -260,6 +260,12 @@ KQ_API long kq_selftest_fastdiv(uint32_t d, uint32_t n_max);
  * low-precision update of shift k to its FP64 solution, for a system with max_term shifts still
  * iterating and the host's divergence flag `discard` */
 KQ_API int kq_selftest_fold_keeps(int discard, int k, int max_term);
+/* host evaluation of the mixed solver's segment-fold coefficients (deferred solution updates of
+ * a multishift system): h holds J <= 32 recorded iterations, h[(j*4 + f)*32 + k] with f = 0 B_k,
+ * 1 Z_k, 2 a_k, and h[(j*4 + 3)*32] = the number of shifts updated in iteration j; out gets 66
+ * doubles: [c_p, c_r[0..32), d_p, d_r[0..32)] such that over the segment
+ * x_k += c_p p_k(start) + sum_j c_r[j] r_j and p_k(end) = d_p p_k(start) + sum_j d_r[j] r_j */
+KQ_API void kq_selftest_seg_coef(const double *h, int J, int k, int keep_x, double *out);
 /* checkerboard index of the neighbour of site s (parity q) at hop h along mu,
  * computed by the kernels' index math on the host */
 KQ_API long kq_neighbour_index(const int lat[4], int q, long s, int mu, int h);

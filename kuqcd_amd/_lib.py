@@ -222,6 +222,7 @@ _PROTOS = {
     "kq_device_count": (C.c_int, []),
     "kq_selftest_fastdiv": (C.c_long, [C.c_uint32, C.c_uint32]),
     "kq_selftest_fold_keeps": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+    "kq_selftest_seg_coef": (None, [_VP, C.c_int, C.c_int, C.c_int, _VP]),
     "kq_neighbour_index": (C.c_long, [C.POINTER(C.c_int), C.c_int, C.c_long, C.c_int, C.c_int]),
 }
 

@@ -28,7 +28,29 @@ template <typename S> struct CgmState {
     double pause2;            // pause when |r|^2 < pause2 * r2max (0: never)
     int nshift, maxTerm, maxTermUpd, iter, maxIter, stop, run, maxTermFold, pause;
     int discard;  // set by the host for one fold: drop x_lo instead of adding it (diverged iteration)
+    // deferred solution updates (multishift systems of the mixed solver): history slot of the
+    // system (-1: x and p of every shift are updated each iteration) and the number of
+    // iterations recorded since the last segment fold
+    int dslot, hist;
 };
+
+// Deferred solution updates. In CGM::Invert every live shift k updates
+//   x_k -= B_k p_k ;  p_k = Z_k r + a_k p_k            (inverter_cgm.cpp:61-69)
+// each iteration: 4 vector passes per shift. Over a segment of J iterations every p_k is
+// p_k(start) plus a combination of the residuals r_1..r_J of the segment, so a system with a
+// history slot keeps only p_0 (the hopping term's source) current and records r and the
+// scalars B_k, Z_k, a_k of each iteration; a segment fold then adds each shift's J updates to
+// its FP64 solution in one pass and rebuilds p_k (k >= 1) from p_k(start) and the recorded
+// residuals. A fold happens whenever the solver folds anyway (reliable update, dropped shift,
+// stop) and at the latest after kHistMax iterations.
+constexpr int kHistMax = 32;
+// recorded scalars per history slot and iteration: B_k, Z_k (after the iteration), a_k, and
+// maxTermUpd in field 3
+constexpr int kHistFields = 4;
+__host__ __device__ inline size_t hist_scalar_index(int dslot, int j, int field, int k)
+{
+    return (((size_t)dslot * kHistMax + j) * kHistFields + field) * kMaxShift + k;
+}
 
 struct CgmLaunch {
     uint32_t Vh;
@@ -36,12 +58,19 @@ struct CgmLaunch {
     uint32_t nblk_dsl;  // blocks of the hopping-term launch (= stride of part_a)
     int nsys;           // system slots (k_cgm_begin scans all of them)
     int nlaunch;        // slots of the live list the per-system kernels cover (>= live systems)
-    int *rlist;         // compact list of live systems (written by k_cgm_begin)
+    int *rlist;         // compact list of live systems this iteration (k_cgm_begin at the start of a
+                        // chunk, then the previous iteration's k_cgm_scalars2)
     int *nact;          // its length
     int *any;           // 1 while any system is live
+    int *rlist_next;    // the list of the next iteration, written by k_cgm_scalars2 (null: none)
+    int *nact_next;
+    int *any_next;
+    int *done;          // completion counter of k_cgm_scalars2's workgroups (0 between launches)
     double *part_a;     // [nsys][nblk_dsl] partial sums of <p, A p>
     double *part_b;     // [nsys][nblk] partial sums of |r|^2
     const size_t *off;  // [nsys] element offset of each system's p/x block
+    double *shist;      // recorded scalars (hist_scalar_index), null without deferred updates
+    void *rhist;        // recorded residuals: vector (dslot * kHistMax + j), 3 * Vh complex each
 };
 
 template <typename S> void cgm_begin(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream);
@@ -53,6 +82,55 @@ void cgm_axpy_norm(CgmState<S> *st, Vec<VR> r, Vec<VR> s, const CgmLaunch &L, hi
 // x_k -= beta_k p_k ; p_k = zeta_k r + alpha_k p_k      (inverter_cgm.cpp:61-69)
 template <typename S, typename VR, typename XR>
 void cgm_xp_update(CgmState<S> *st, XR *x, VR *p, Vec<VR> r, const CgmLaunch &L, hipStream_t stream);
+
+// ---- deferred solution updates (segment fold) of the listed systems with a history slot:
+// coefficients of each shift k < maxTermFold from the recorded scalars (coef: per slot and
+// shift, [c_p, c_r[0..kHistMax)] for x and [d_p, d_r[0..kHistMax)] for p), then
+//   x_hi[k] += c_p p_k(start) + sum_j c_r[j] r_j   (if fold_keeps(discard, k, maxTerm))
+//   p_k      = d_p p_k(start) + sum_j d_r[j] r_j   (1 <= k < maxTerm)
+// with p_0(start) in p0snap[dslot]; then p0snap = p_0 and hist = 0 (seg_reset, also at the
+// start and restart of a system). kmax bounds maxTermFold over the listed systems.
+constexpr int kCoefStride = 2 * (kHistMax + 1);
+// One coefficient pair of shift k over the J recorded iterations of one history slot (h points
+// at the slot's first scalar, hist_scalar_index(dslot, 0, 0, 0)): for m = -1 the start vector
+// p_k(start), for 0 <= m < kHistMax the recorded residual r_m. cx is its coefficient in the
+// segment's update of x_k, cp its coefficient in p_k at the end of the segment. Shift k was
+// updated by the first Jk iterations (k < maxTermUpd_j); within them
+//   x_k += -B_j p_k,j ,   p_k,j+1 = Z_j r_j + a_j p_k,j ,
+// so a term that enters p_k with weight w (1 for p_k(start) at j = 0, Z_m for r_m at j = m + 1)
+// adds -B_j w to cx at every later iteration j and is scaled by a_j. keep_x = false (a
+// diverged segment) drops the x part.
+__host__ __device__ inline void seg_coefficient(const double *h, int J, int k, int m, bool keep_x, double &cx,
+                                                double &cp)
+{
+    const size_t step = (size_t)kHistFields * kMaxShift;
+    int Jk = 0;
+    while (Jk < J && Jk < kHistMax && k < (int)h[(size_t)Jk * step + 3 * kMaxShift])
+        ++Jk;
+    cx = 0.0;
+    cp = 0.0;
+    if (m >= Jk)
+        return;  // r_m came after shift k was dropped
+    // the start vector is in p_k,0 (iterations from 0 on see it); r_m enters p_k,m+1
+    double w = (m < 0) ? 1.0 : h[(size_t)m * step + kMaxShift + k];
+    double sx = 0.0;
+    for (int j = m + 1; j < Jk; ++j) {
+        sx -= h[(size_t)j * step + k] * w;
+        w *= h[(size_t)j * step + 2 * kMaxShift + k];
+    }
+    cx = keep_x ? sx : 0.0;
+    cp = w;
+}
+
+template <typename S>
+void seg_coef(const CgmState<S> *st, const double *shist, double *coef, const int *list, const int *n, int nsys,
+              hipStream_t stream);
+template <typename S, typename VR>
+void seg_fold(const CgmState<S> *st, const double *coef, double *x_hi, VR *p, const VR *p0snap, const VR *rhist,
+              const size_t *off, uint32_t Vh, const int *list, const int *n, int nsys, int kmax, hipStream_t stream);
+template <typename S, typename VR>
+void seg_reset(CgmState<S> *st, const VR *p, VR *p0snap, const size_t *off, uint32_t Vh, const int *list, const int *n,
+               int nsys, hipStream_t stream);
 
 // ---- start of listed systems (list[0..*n)): rr = r2max = sum of part_b,
 // scalars as inverter_cgm.cpp:30-39, stop = pause = 0 (stop = 1 if nshift <= 0)

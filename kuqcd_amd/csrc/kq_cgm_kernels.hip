@@ -57,7 +57,7 @@ template <typename S> void cgm_begin(CgmState<S> *st, const CgmLaunch &L, hipStr
 // reference keeps its type (the denominator is formed in double for S=float).
 template <typename S>
 __global__ void __launch_bounds__(256) k_cgm_scalars1(CgmState<S> *st, const int *rlist, const int *nact,
-                                                      const double *part, uint32_t nblk)
+                                                      const double *part, uint32_t nblk, double *shist)
 {
 #pragma clang fp contract(off)
     const int slot = blockIdx.x;
@@ -65,55 +65,97 @@ __global__ void __launch_bounds__(256) k_cgm_scalars1(CgmState<S> *st, const int
         return;
     const int r = rlist[slot];
     const double pAp_d = block_reduce_fixed(part + (size_t)r * nblk, nblk);
-    if (threadIdx.x != 0)
-        return;
     CgmState<S> &c = st[r];
-    const S pAp = (S)pAp_d;
-    const S B0 = -c.rr / pAp;
-    c.B[0] = B0;
-    for (int j = 1; j < c.maxTerm; ++j) {
-        const S num = c.Bm1 * c.Zm1[j];
-        const S t1 = B0 * c.a[0] * (c.Zm1[j] - c.Z[j]);
-        const S t2 = c.Zm1[j] * c.Bm1;
+    // every thread reads the values the shifted updates share before any thread writes
+    const S B0 = -c.rr / (S)pAp_d;
+    const S Bm1 = c.Bm1, a0 = c.a[0];
+    const int mt = c.maxTerm;
+    const bool rec = shist != nullptr && c.dslot >= 0 && c.hist < kHistMax;
+    double *h = rec ? shist + hist_scalar_index(c.dslot, c.hist, 0, 0) : nullptr;
+    __syncthreads();
+    // one shifted system per thread; each touches only its own entries
+    for (int j = 1 + (int)threadIdx.x; j < mt; j += blockDim.x) {
+        const S zm1 = c.Zm1[j], z = c.Z[j];
+        const S num = Bm1 * zm1;
+        const S t1 = B0 * a0 * (zm1 - z);
+        const S t2 = zm1 * Bm1;
         const S t3 = c.shift[j] * B0;
         const S rr1 = (S)((double)num / ((double)t1 + (double)t2 * (1.0 - (double)t3)));
-        c.Zm1[j] = c.Z[j];
-        c.Z[j] = c.Z[j] * rr1;
-        c.B[j] = B0 * rr1;
+        const S znew = z * rr1, bnew = B0 * rr1;
+        c.Zm1[j] = z;
+        c.Z[j] = znew;
+        c.B[j] = bnew;
+        if (rec) {
+            h[j] = (double)bnew;
+            h[kMaxShift + j] = (double)znew;
+        }
     }
-    c.Bm1 = B0;
+    if (threadIdx.x == 0) {
+        c.B[0] = B0;
+        c.Bm1 = B0;
+        if (rec) {
+            h[0] = (double)B0;
+            h[kMaxShift] = (double)c.Z[0];
+        }
+    }
 }
 
 template <typename S> void cgm_scalars1(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_cgm_scalars1<S>, dim3(L.nlaunch), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_a,
-                       L.nblk_dsl);
+                       L.nblk_dsl, L.shist);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
 // ------------------------------------------------------------ scalars 2
 // |r|^2 new, alpha recurrences, drop converged shifts from the end,
 // iteration count and stop flag (inverter_cgm.cpp:58-78); pause flag for the
-// reliable-update schedule.
+// reliable-update schedule. The last of its workgroups to finish writes the next
+// iteration's list of live systems (what k_cgm_begin does at the start of a chunk),
+// so an iteration needs no separate list kernel.
 template <typename S>
 __global__ void __launch_bounds__(256) k_cgm_scalars2(CgmState<S> *st, const int *rlist, const int *nact,
-                                                      const double *part, uint32_t nblk)
+                                                      const double *part, uint32_t nblk, double *shist, int nsys,
+                                                      int *rlist_next, int *nact_next, int *any_next, int *done)
 {
 #pragma clang fp contract(off)
     const int slot = blockIdx.x;
-    if (*nact <= slot)
+    const int n_live = *nact;
+    if (n_live <= slot) {
+        // nothing is live this iteration, and no system becomes live inside a chunk
+        if (n_live == 0 && slot == 0 && threadIdx.x == 0 && rlist_next != nullptr) {
+            *nact_next = 0;
+            *any_next = 0;
+        }
         return;
+    }
     const int r = rlist[slot];
     const double l2_d = block_reduce_fixed(part + (size_t)r * nblk, nblk);
+    CgmState<S> &c = st[r];
+    // every thread reads the values the shifted updates share before thread 0 writes them
+    const S lambda2 = (S)l2_d;
+    const S a0 = lambda2 / c.rr;
+    const S B0 = c.B[0];
+    const int mt = c.maxTerm;
+    const bool rec = shist != nullptr && c.dslot >= 0 && c.hist < kHistMax;
+    double *h = rec ? shist + hist_scalar_index(c.dslot, c.hist, 2, 0) : nullptr;
+    __syncthreads();
+    for (int j = 1 + (int)threadIdx.x; j < mt; j += blockDim.x) {
+        const S aj = a0 * c.Z[j] * c.B[j] / (c.Zm1[j] * B0);
+        c.a[j] = aj;
+        if (rec)
+            h[j] = (double)aj;
+    }
     if (threadIdx.x != 0)
         return;
-    CgmState<S> &c = st[r];
-    const S lambda2 = (S)l2_d;
-    c.a[0] = lambda2 / c.rr;
+    c.a[0] = a0;
     c.rr = lambda2;
-    for (int j = 1; j < c.maxTerm; ++j)
-        c.a[j] = c.a[0] * c.Z[j] * c.B[j] / (c.Zm1[j] * c.B[0]);
-    c.maxTermUpd = c.maxTerm;
+    c.maxTermUpd = mt;
+    if (rec) {
+        h[0] = (double)a0;
+        h[kMaxShift] = (double)mt;
+        c.hist += 1;
+    }
     // drop converged shifts from the end of the list (inverter_cgm.cpp:71-76)
     while (c.maxTerm > 0) {
         const S lambda = c.Z[c.maxTerm - 1] * c.Z[c.maxTerm - 1] * lambda2;
@@ -126,12 +168,29 @@ __global__ void __launch_bounds__(256) k_cgm_scalars2(CgmState<S> *st, const int
         c.stop = 1;
     if (c.pause2 > 0.0 && l2_d < c.pause2 * c.r2max)
         c.pause = 1;
+    if (rlist_next == nullptr)
+        return;
+    // the last workgroup (of the n_live that get here) sees every system's stop and pause flags
+    __threadfence();
+    if (atomicAdd(done, 1) != n_live - 1)
+        return;
+    __threadfence();
+    int n = 0;
+    for (int q = 0; q < nsys; ++q) {
+        const int live = (st[q].stop == 0) && (st[q].pause == 0);
+        st[q].run = live;
+        if (live)
+            rlist_next[n++] = q;
+    }
+    *nact_next = n;
+    *any_next = n > 0;
+    *done = 0;
 }
 
 template <typename S> void cgm_scalars2(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_cgm_scalars2<S>, dim3(L.nlaunch), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_b,
-                       L.nblk);
+                       L.nblk, L.shist, L.nsys, L.rlist_next, L.nact_next, L.any_next, L.done);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
@@ -139,7 +198,7 @@ template <typename S> void cgm_scalars2(CgmState<S> *st, const CgmLaunch &L, hip
 template <typename S, typename VR>
 __global__ void __launch_bounds__(kBlasNT) k_axpy_norm(const CgmState<S> *st, Vec<VR> r, Vec<VR> s,
                                                        const int *rlist, const int *nact, double *part,
-                                                       uint32_t nblk)
+                                                       uint32_t nblk, c2_t<VR> *rhist)
 {
     using V2 = c2_t<VR>;
     __shared__ double lds[kBlasNT / kWave];
@@ -161,6 +220,13 @@ __global__ void __launch_bounds__(kBlasNT) k_axpy_norm(const CgmState<S> *st, Ve
             acc += (double)rv[c].x * (double)rv[c].x + (double)rv[c].y * (double)rv[c].y;
         }
         r.store(rh, i, rv);
+        const int ds = st[rh].dslot, hj = st[rh].hist;
+        if (rhist != nullptr && ds >= 0 && hj < kHistMax) {
+            V2 *h = rhist + ((size_t)ds * kHistMax + hj) * 3 * r.Vh + i;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                st_nt(h + (size_t)c * r.Vh, rv[c]);
+        }
     }
     const double tot = block_sum<kBlasNT>(acc, lds);
     if (threadIdx.x == 0)
@@ -171,7 +237,7 @@ template <typename S, typename VR>
 void cgm_axpy_norm(CgmState<S> *st, Vec<VR> r, Vec<VR> s, const CgmLaunch &L, hipStream_t stream)
 {
     hipLaunchKernelGGL((k_axpy_norm<S, VR>), dim3(L.nblk, L.nlaunch), dim3(kBlasNT), 0, stream, st, r, s, L.rlist,
-                       L.nact, L.part_b, L.nblk);
+                       L.nact, L.part_b, L.nblk, static_cast<c2_t<VR> *>(L.rhist));
     KQ_HIP_CHECK(hipGetLastError());
 }
 
@@ -191,7 +257,10 @@ __global__ void __launch_bounds__(kBlasNT) k_xp_update(const CgmState<S> *st, XR
     if (i >= Vh)
         return;
     const CgmState<S> &c = st[rh];
-    const int m = c.maxTermUpd;
+    // a system with deferred solution updates keeps only p_0 current (its x_k and p_k, k >= 1,
+    // are brought up to date by the next segment fold)
+    const bool defer = c.dslot >= 0;
+    const int m = defer ? 1 : c.maxTermUpd;
     const size_t o = off[rh];
     V2 rv[3];
     r.load(rh, i, rv);
@@ -207,10 +276,12 @@ __global__ void __launch_bounds__(kBlasNT) k_xp_update(const CgmState<S> *st, XR
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
             const V2 pv = ld_nt(pp + (size_t)cc * Vh);
-            X2 xv = ld_nt(xx + (size_t)cc * Vh);
-            xv.x -= bk * (XR)pv.x;
-            xv.y -= bk * (XR)pv.y;
-            st_nt(xx + (size_t)cc * Vh, xv);
+            if (!defer) {
+                X2 xv = ld_nt(xx + (size_t)cc * Vh);
+                xv.x -= bk * (XR)pv.x;
+                xv.y -= bk * (XR)pv.y;
+                st_nt(xx + (size_t)cc * Vh, xv);
+            }
             V2 np;
             np.x = zk * rv[cc].x + ak * pv.x;
             np.y = zk * rv[cc].y + ak * pv.y;
@@ -258,6 +329,7 @@ __global__ void __launch_bounds__(256) k_cgm_init_list(CgmState<S> *st, const in
     c.iter = 0;
     c.pause = 0;
     c.discard = 0;
+    c.hist = 0;
     // the reference's do-while runs at least one iteration even for max_iter <= 0
     c.stop = (c.nshift <= 0) ? 1 : 0;
     c.run = 0;
@@ -385,6 +457,7 @@ template <typename S> __global__ void k_cgm_restart_list(CgmState<S> *st, const 
     }
     c.r2max = (double)c.rr;
     c.pause = 0;
+    c.hist = 0;
 }
 
 template <typename S> void cgm_restart_list(CgmState<S> *st, const int *list, const int *n, int nsys, hipStream_t stream)
@@ -438,6 +511,145 @@ void rel_fold(CgmState<S> *st, double *x_hi, VR *x_lo, const size_t *off, uint32
     const size_t n3 = (size_t)3 * Vh;
     hipLaunchKernelGGL((k_rel_fold<S, VR>), dim3((unsigned)((n3 + kBlasNT - 1) / kBlasNT), nsys), dim3(kBlasNT), 0,
                        stream, st, x_hi, x_lo, off, Vh, list, n);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ deferred solution updates
+// One workgroup per listed system, one thread per (shift k < maxTermFold, term m): the start
+// vector (m = -1) or recorded residual m; each forms its pair of coefficients in O(J).
+template <typename S>
+__global__ void __launch_bounds__(256) k_seg_coef(const CgmState<S> *st, const double *shist, double *coef,
+                                                  const int *list, const int *n)
+{
+    const int slot = blockIdx.x;
+    if (*n <= slot)
+        return;
+    const CgmState<S> &c = st[list[slot]];
+    if (c.dslot < 0)
+        return;
+    const double *h = shist + hist_scalar_index(c.dslot, 0, 0, 0);
+    const int terms = kHistMax + 1;
+    for (int idx = threadIdx.x; idx < c.maxTermFold * terms; idx += blockDim.x) {
+        const int k = idx / terms, m = idx % terms - 1;
+        double cx, cp;
+        seg_coefficient(h, c.hist, k, m, fold_keeps(c.discard, k, c.maxTerm), cx, cp);
+        double *o = coef + ((size_t)c.dslot * kMaxShift + k) * kCoefStride;
+        o[1 + m] = cx;                // m = -1: o[0]
+        o[kHistMax + 2 + m] = cp;     // m = -1: o[kHistMax + 1]
+    }
+}
+
+template <typename S>
+void seg_coef(const CgmState<S> *st, const double *shist, double *coef, const int *list, const int *n, int nsys,
+              hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_seg_coef<S>, dim3(nsys), dim3(256), 0, stream, st, shist, coef, list, n);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// One thread per complex entry of a vector; blockIdx.x = shift (fastest, so the blocks of all
+// shifts of one site range run together and read the recorded residuals from L2).
+template <typename S, typename VR>
+__global__ void __launch_bounds__(kBlasNT) k_seg_fold(const CgmState<S> *st, const double *coef, double *x_hi, VR *p,
+                                                      const VR *p0snap, const VR *rhist, const size_t *off,
+                                                      uint32_t Vh, const int *list, const int *n)
+{
+    using V2 = c2_t<VR>;
+    const int k = blockIdx.x, slot = blockIdx.z;
+    if (*n <= slot)
+        return;
+    const int rh = list[slot];
+    const CgmState<S> &c = st[rh];
+    if (c.dslot < 0 || k >= c.maxTermFold)
+        return;
+    const size_t n3 = (size_t)3 * Vh;
+    const size_t e = (size_t)blockIdx.y * kBlasNT + threadIdx.x;
+    if (e >= n3)
+        return;
+    const bool keep_x = fold_keeps(c.discard, k, c.maxTerm);
+    const bool upd_p = k >= 1 && k < c.maxTerm;
+    if (!keep_x && !upd_p)
+        return;
+    const int J = c.hist < kHistMax ? c.hist : kHistMax;
+    const double *cf = coef + ((size_t)c.dslot * kMaxShift + k) * kCoefStride;
+    const size_t pe = off[rh] + (size_t)k * n3 + e;
+    const V2 ps = (k == 0) ? reinterpret_cast<const V2 *>(p0snap)[(size_t)c.dslot * n3 + e]
+                           : ld_nt(reinterpret_cast<const V2 *>(p) + pe);
+    double xr = cf[0] * (double)ps.x, xi = cf[0] * (double)ps.y;
+    double pr = cf[kHistMax + 1] * (double)ps.x, pi = cf[kHistMax + 1] * (double)ps.y;
+    const V2 *rh_base = reinterpret_cast<const V2 *>(rhist) + (size_t)c.dslot * kHistMax * n3 + e;
+    for (int j = 0; j < J; ++j) {
+        const V2 rv = rh_base[(size_t)j * n3];
+        xr += cf[1 + j] * (double)rv.x;
+        xi += cf[1 + j] * (double)rv.y;
+        pr += cf[kHistMax + 2 + j] * (double)rv.x;
+        pi += cf[kHistMax + 2 + j] * (double)rv.y;
+    }
+    if (keep_x) {
+        double2 *xh = reinterpret_cast<double2 *>(x_hi) + pe;
+        double2 h = ld_nt(xh);
+        h.x += xr;
+        h.y += xi;
+        // the unshifted solution is read next by the FP64 hopping term of a reliable update
+        if (k == 0)
+            *xh = h;
+        else
+            st_nt(xh, h);
+    }
+    if (upd_p) {
+        V2 np;
+        np.x = (VR)pr;
+        np.y = (VR)pi;
+        st_nt(reinterpret_cast<V2 *>(p) + pe, np);
+    }
+}
+
+template <typename S, typename VR>
+void seg_fold(const CgmState<S> *st, const double *coef, double *x_hi, VR *p, const VR *p0snap, const VR *rhist,
+              const size_t *off, uint32_t Vh, const int *list, const int *n, int nsys, int kmax, hipStream_t stream)
+{
+    if (kmax <= 0)
+        return;
+    const size_t n3 = (size_t)3 * Vh;
+    hipLaunchKernelGGL((k_seg_fold<S, VR>), dim3(kmax, (unsigned)((n3 + kBlasNT - 1) / kBlasNT), nsys), dim3(kBlasNT),
+                       0, stream, st, coef, x_hi, p, p0snap, rhist, off, Vh, list, n);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+// p0snap = p_0 and hist = 0 for the listed systems with a history slot
+template <typename S, typename VR>
+__global__ void __launch_bounds__(kBlasNT) k_seg_reset(CgmState<S> *st, const VR *p, VR *p0snap, const size_t *off,
+                                                       uint32_t Vh, const int *list, const int *n)
+{
+    using V2 = c2_t<VR>;
+    const int slot = blockIdx.y;
+    if (*n <= slot)
+        return;
+    const int rh = list[slot];
+    const int ds = st[rh].dslot;
+    if (ds < 0)
+        return;
+    const size_t n3 = (size_t)3 * Vh;
+    const size_t e = (size_t)blockIdx.x * kBlasNT + threadIdx.x;
+    if (e < n3)
+        reinterpret_cast<V2 *>(p0snap)[(size_t)ds * n3 + e] = reinterpret_cast<const V2 *>(p)[off[rh] + e];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st[rh].hist = 0;
+        // with one shift left the system goes back to per-iteration updates of x_0 and p_0 (its
+        // x_lo, never written while deferred, is zero): the deferral saves nothing there, and
+        // x_0 then follows the FP32 p_0 the operator was applied to, as in CGM::Invert
+        if (st[rh].maxTerm <= 1)
+            st[rh].dslot = -1;
+    }
+}
+
+template <typename S, typename VR>
+void seg_reset(CgmState<S> *st, const VR *p, VR *p0snap, const size_t *off, uint32_t Vh, const int *list, const int *n,
+               int nsys, hipStream_t stream)
+{
+    const size_t n3 = (size_t)3 * Vh;
+    hipLaunchKernelGGL((k_seg_reset<S, VR>), dim3((unsigned)((n3 + kBlasNT - 1) / kBlasNT), nsys), dim3(kBlasNT), 0,
+                       stream, st, p, p0snap, off, Vh, list, n);
     KQ_HIP_CHECK(hipGetLastError());
 }
 
@@ -529,7 +741,9 @@ void reduce_partials(const double *part, uint32_t nblk, int nrhs, double *out, h
                                    hipStream_t);                                                              \
     template void rel_set_rr<S>(CgmState<S> *, const int *, const int *, int, const double *, uint32_t,       \
                                 hipStream_t);                                                              \
-    template void cgm_restart_list<S>(CgmState<S> *, const int *, const int *, int, hipStream_t);
+    template void cgm_restart_list<S>(CgmState<S> *, const int *, const int *, int, hipStream_t);          \
+    template void seg_coef<S>(const CgmState<S> *, const double *, double *, const int *, const int *, int,   \
+                              hipStream_t);
 KQ_INST_S(double)
 KQ_INST_S(float)
 
@@ -539,7 +753,11 @@ KQ_INST_S(float)
     template void rel_fold<S, VR>(CgmState<S> *, double *, VR *, const size_t *, uint32_t, const int *,        \
                                   const int *, int, hipStream_t);                                             \
     template void copy_r_to_p<S, VR>(const CgmState<S> *, Vec<VR>, VR *, const size_t *, const int *,          \
-                                     const int *, int, hipStream_t);
+                                     const int *, int, hipStream_t);                                          \
+    template void seg_fold<S, VR>(const CgmState<S> *, const double *, double *, VR *, const VR *, const VR *,  \
+                                  const size_t *, uint32_t, const int *, const int *, int, int, hipStream_t);  \
+    template void seg_reset<S, VR>(CgmState<S> *, const VR *, VR *, const size_t *, uint32_t, const int *,     \
+                                   const int *, int, hipStream_t);
 KQ_INST_SV(double, double)
 KQ_INST_SV(float, float)
 KQ_INST_SV(double, float)

@@ -443,6 +443,14 @@ int kq_selftest_fold_keeps(int discard, int k, int max_term)
     return fold_keeps(discard, k, max_term) ? 1 : 0;
 }
 
+void kq_selftest_seg_coef(const double *h, int J, int k, int keep_x, double *out)
+{
+    if (h == nullptr || out == nullptr || k < 0 || k >= kMaxShift)
+        return;
+    for (int m = -1; m < kHistMax; ++m)
+        seg_coefficient(h, J, k, m, keep_x != 0, out[1 + m], out[kHistMax + 2 + m]);
+}
+
 long kq_neighbour_index(const int lat[4], int q, long s, int mu, int h)
 {
     const Geom g = make_geom(lat);

@@ -148,7 +148,8 @@ static std::vector<double> true_res2(int slot, int parity, const double *b, cons
     a.aux = xv;
     a.sigma = sig.as<double>();
     a.partial = part.as<double>();
-    launch_dslash(a, EPI_AXMY_DOT, nr, nt, c.stream, cap);
+    // one vector per thread for the fused epilogue (its two-vector FP64 form is slower than two passes)
+    launch_dslash(a, EPI_AXMY_DOT, 1, nt, c.stream, tuned_cap(slot, KQ_MODE_FP64, 1, nt));
     const uint32_t nb = blas_nblk(g.Vh);
     resid2_partial(b, ax.as<double>(), nl, list.as<int>(), g.Vh, part.as<double>(), nb, c.stream);
     const std::vector<double> all = reduce_host(part.as<double>(), nb, nk, c.stream);
@@ -163,6 +164,12 @@ static bool debug_on()
 {
     static const bool on = getenv("KQ_DEBUG") != nullptr;
     return on;
+}
+
+static bool deferred_updates()
+{
+    const char *e = getenv("KQ_DEFERRED");
+    return !(e != nullptr && e[0] == '0');
 }
 
 // autotuned right-hand-side blocking for 0..min(max_sys, 16) live systems
@@ -262,6 +269,10 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
     CgmRunner<double, float, LF, LL> run(P.g, P.parity, fat, lng, max_sys, (size_t)nrhs * ns + (size_t)nrhs * (ns - 1),
                                          c.stream, tune);
     run.enable_reliable(link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.delta);
+    // the multishift systems fold their solution updates per segment instead of updating every
+    // shift's x and p each iteration (KQ_DEFERRED=0 switches this off)
+    if (deferred_updates())
+        run.enable_deferred(nrhs);
     std::vector<int> sys_of(nrhs);
     std::vector<int> rhs_of(max_sys, -1);
     std::vector<int> forked((size_t)nrhs * ns, -1);

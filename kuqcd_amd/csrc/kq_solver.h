@@ -29,6 +29,10 @@
 #include "kq_dslash.h"
 #include "kq_layout.h"
 
+#ifndef KQ_LIST_IN_SCALARS2
+#define KQ_LIST_IN_SCALARS2 1
+#endif
+
 namespace kq {
 
 // Device-memory cache: released buffers are kept by size and handed out again to
@@ -249,7 +253,12 @@ public:
         // launch_dslash accepts fit either
         part_a_.alloc(sizeof(double) * max_sys * dslash_nblk(g.Vh, kMinDslashNT));
         part_b_.alloc(sizeof(double) * max_sys * nblk_);
-        ints_.alloc(sizeof(int) * (2 * max_sys + 4));
+        // two live lists (iterations alternate between them), the staged system list, and the
+        // completion counter of k_cgm_scalars2
+        ints_.alloc(sizeof(int) * (3 * max_sys + 8));
+        KQ_HIP_CHECK(hipMemsetAsync(ints_.as<void>(), 0, ints_.bytes(), st_));
+        if (tune_.chunk % 2 != 0)
+            tune_.chunk += 1;  // iterations run in pairs, one per live list
         sigma_.alloc(sizeof(double) * max_sys);
         off_.alloc(sizeof(size_t) * max_sys);
         off_h_.assign(max_sys, 0);
@@ -286,6 +295,24 @@ public:
         ax64_.alloc(sizeof(double2) * max_sys_ * n3_);
     }
 
+    // Deferred solution updates (reliable mode) for up to `slots` multishift systems: each
+    // records its residuals and recurrence scalars, keeps only p_0 current, and has its x_k
+    // and p_k brought up to date by segment folds (kq_cgm_kernels.h)
+    void enable_deferred(int slots)
+    {
+        if (!reliable_)
+            throw std::logic_error("deferred solution updates need the reliable mode");
+        if (tune_.chunk > kHistMax)
+            throw std::invalid_argument("deferred solution updates: chunk longer than the history");
+        ndslot_ = std::max(0, slots);
+        if (ndslot_ == 0)
+            return;
+        shist_.alloc(sizeof(double) * hist_scalar_index(ndslot_, 0, 0, 0));
+        rhist_.alloc(sizeof(V2) * (size_t)ndslot_ * kHistMax * n3_);
+        p0snap_.alloc(sizeof(V2) * (size_t)ndslot_ * n3_);
+        coef_.alloc(sizeof(double) * (size_t)ndslot_ * kMaxShift * kCoefStride);
+    }
+
     // Register a system; it starts at the next run() step. shifts: reference
     // convention (absolute shifts[0], shifts[k>=1] relative to it); tolk: absolute
     // squared target per shift. b: device source, FP64 SoA in reliable mode, VR
@@ -309,6 +336,8 @@ public:
         c.maxIter = max_iter;
         c.stop = 1;  // until started
         c.pause2 = reliable_ ? pause2_ : 0.0;
+        c.dslot = (nshift > 1 && dslot_used_ < ndslot_) ? dslot_used_++ : -1;
+        c.hist = 0;
         for (int k = 0; k < nshift; ++k) {
             c.shift[k] = shifts[k];
             c.tolk[k] = tolk[k];
@@ -351,13 +380,17 @@ public:
             int nlive = 0;
             for (int sys = 0; sys < nsys_; ++sys)
                 nlive += (!state_h_[sys].stop && !state_h_[sys].pause) ? 1 : 0;
+            // the chunk's first live list comes from k_cgm_begin; each iteration's
+            // k_cgm_scalars2 writes the next one
             if (direct_launches()) {
                 const int nr = tune_.pick_nr(nlive);
+                cgm_begin<S>(state_.as<CgmState<S>>(), launch(1, 0), st_);
                 for (int c = 0; c < tune_.chunk; ++c)
-                    iteration(nr, std::max(1, (nlive + nr - 1) / nr));
+                    iteration(nr, std::max(1, (nlive + nr - 1) / nr), c & 1);
             } else {
-                hipGraphExec_t gx = graph_for(nlive);
-                for (int c = 0; c < tune_.chunk; ++c)
+                hipGraphExec_t gx = graph_for(nlive);  // captured before anything is queued
+                cgm_begin<S>(state_.as<CgmState<S>>(), launch(1, 0), st_);
+                for (int c = 0; c < tune_.chunk; c += 2)
                     KQ_HIP_CHECK(hipGraphLaunch(gx, st_));
             }
             const auto tb = clk::now();
@@ -396,7 +429,9 @@ public:
                     reliable_ && (c.pause || (double)c.rr < delta_ * delta_ * c.r2max || diverging);
                 rr_iter[sys] = diverging ? 0.0 : (double)c.rr;
                 c.discard = diverging ? 1 : 0;  // the updates since the last fold are not trusted
-                if (reliable_ && (need_rel || dropped)) {
+                // a recorded segment is folded before the next chunk could overrun the history
+                const bool hist_full = c.dslot >= 0 && c.hist + tune_.chunk > kHistMax;
+                if (reliable_ && (need_rel || dropped || hist_full)) {
                     c.maxTermFold = last_fold_[sys];
                     fold.push_back(sys);
                 }
@@ -422,6 +457,9 @@ public:
                 copy_r_to_p<S, VR>(state_.as<CgmState<S>>(), vec(r_), p_.as<VR>(), off_.as<size_t>(), list(),
                                    list_n(), list_grid(), st_);
                 cgm_restart_list<S>(state_.as<CgmState<S>>(), list(), list_n(), list_grid(), st_);
+                if (ndslot_ > 0)
+                    seg_reset<S, VR>(state_.as<CgmState<S>>(), p_.as<VR>(), p0snap_.as<VR>(), off_.as<size_t>(), g_.Vh,
+                                     list(), list_n(), list_grid(), st_);
                 restarts_ += (int)restart.size();
                 download_states();
                 if (dbg)
@@ -456,13 +494,15 @@ public:
 private:
     Vec<VR> vec(const DevBuf &b) const { return Vec<VR>{b.as<V2>(), g_.Vh, n3_}; }
     Vec<VR> vec_p0() const { return Vec<VR>{p_.as<V2>(), g_.Vh, 0, off_.as<size_t>()}; }
-    int *rlist() const { return ints_.as<int>(); }
-    int *nact() const { return ints_.as<int>() + max_sys_; }
-    int *any() const { return ints_.as<int>() + max_sys_ + 1; }
+    // live list of iterations with phase 0 (the first of a chunk) and phase 1
+    int *rlist(int ph = 0) const { return ints_.as<int>() + (ph ? 2 * max_sys_ + 3 : 0); }
+    int *nact(int ph = 0) const { return ints_.as<int>() + (ph ? 3 * max_sys_ + 3 : max_sys_); }
+    int *any(int ph = 0) const { return ints_.as<int>() + (ph ? 3 * max_sys_ + 4 : max_sys_ + 1); }
     int *list() const { return ints_.as<int>() + max_sys_ + 2; }
     int *list_n() const { return ints_.as<int>() + 2 * max_sys_ + 2; }
+    int *done_counter() const { return ints_.as<int>() + 3 * max_sys_ + 5; }
 
-    CgmLaunch launch(int nlaunch) const
+    CgmLaunch launch(int nlaunch, int ph = 0) const
     {
         CgmLaunch L;
         L.Vh = g_.Vh;
@@ -470,12 +510,26 @@ private:
         L.nblk_dsl = nblk_dsl_;
         L.nsys = max_sys_;
         L.nlaunch = std::min(std::max(nlaunch, 1), max_sys_);
-        L.rlist = rlist();
-        L.nact = nact();
-        L.any = any();
+#if KQ_LIST_IN_SCALARS2
+        L.rlist = rlist(ph);
+        L.nact = nact(ph);
+        L.any = any(ph);
+        L.rlist_next = rlist(1 - ph);
+        L.nact_next = nact(1 - ph);
+        L.any_next = any(1 - ph);
+#else
+        (void)ph;
+        L.rlist = rlist(0);
+        L.nact = nact(0);
+        L.any = any(0);
+        L.rlist_next = L.nact_next = L.any_next = nullptr;
+#endif
+        L.done = done_counter();
         L.part_a = part_a_.as<double>();
         L.part_b = part_b_.as<double>();
         L.off = off_.as<size_t>();
+        L.shist = ndslot_ > 0 ? shist_.as<double>() : nullptr;
+        L.rhist = ndslot_ > 0 ? rhist_.as<void>() : nullptr;
         return L;
     }
 
@@ -513,7 +567,10 @@ private:
     }
     int list_grid() const { return list_size_; }
 
-    // ax64 = A_0 x_hi[.][0] for the n listed systems (FP64 links, read once for up to 4 systems)
+    // ax64 = A_0 x_hi[.][0] for the n listed systems (FP64 links). The first hopping term reads
+    // the links once for two systems; the second, with the fused sigma x - D tmp and <x, Ax>
+    // epilogue, runs one system per thread: with two its FP64 registers cost more than a
+    // second pass over the links (478 us against 2 x 195 us at 32^4)
     void apply_a64_x0(int n)
     {
         const int nr = n >= 2 ? 2 : 1;
@@ -538,7 +595,7 @@ private:
         a.aux = x0;
         a.sigma = sigma_.as<double>();
         a.partial = part_a_.as<double>();
-        launch_dslash(a, EPI_AXMY_DOT, nr, tune_.nt64, st_, cap);
+        launch_dslash(a, EPI_AXMY_DOT, 1, tune_.nt64, st_, tune_.cap64[0]);
     }
 
     void start_pending()
@@ -570,6 +627,9 @@ private:
         copy_r_to_p<S, VR>(state_.as<CgmState<S>>(), vec(r_), p_.as<VR>(), off_.as<size_t>(), list(), list_n(),
                            list_grid(), st_);
         cgm_init_list<S>(state_.as<CgmState<S>>(), list(), list_n(), list_grid(), part_b_.as<double>(), nblk_, st_);
+        if (ndslot_ > 0)
+            seg_reset<S, VR>(state_.as<CgmState<S>>(), p_.as<VR>(), p0snap_.as<VR>(), off_.as<size_t>(), g_.Vh,
+                             list(), list_n(), list_grid(), st_);
         pending_.clear();
         pending_x0_.clear();
         download_states();
@@ -579,9 +639,30 @@ private:
     {
         upload_states();  // maxTermFold and cleared pause flags
         if (!fold.empty()) {
-            set_list(fold);
-            rel_fold<S, VR>(state_.as<CgmState<S>>(), x_hi_.as<double>(), x_.as<VR>(), off_.as<size_t>(), g_.Vh,
-                            list(), list_n(), list_grid(), st_);
+            std::vector<int> plain, seg;
+            int kmax = 0;
+            for (int sys : fold) {
+                if (state_h_[sys].dslot >= 0) {
+                    seg.push_back(sys);
+                    kmax = std::max(kmax, state_h_[sys].maxTermFold);
+                } else {
+                    plain.push_back(sys);
+                }
+            }
+            if (!plain.empty()) {
+                set_list(plain);
+                rel_fold<S, VR>(state_.as<CgmState<S>>(), x_hi_.as<double>(), x_.as<VR>(), off_.as<size_t>(), g_.Vh,
+                                list(), list_n(), list_grid(), st_);
+            }
+            if (!seg.empty()) {
+                set_list(seg);
+                CgmState<S> *stp = state_.as<CgmState<S>>();
+                seg_coef<S>(stp, shist_.as<double>(), coef_.as<double>(), list(), list_n(), list_grid(), st_);
+                seg_fold<S, VR>(stp, coef_.as<double>(), x_hi_.as<double>(), p_.as<VR>(), p0snap_.as<VR>(),
+                                rhist_.as<VR>(), off_.as<size_t>(), g_.Vh, list(), list_n(), list_grid(), kmax, st_);
+                seg_reset<S, VR>(stp, p_.as<VR>(), p0snap_.as<VR>(), off_.as<size_t>(), g_.Vh, list(), list_n(),
+                                 list_grid(), st_);
+            }
             for (int sys : fold)
                 last_fold_[sys] = state_h_[sys].stop ? 0 : std::max(0, state_h_[sys].maxTerm);
         }
@@ -596,21 +677,24 @@ private:
         download_states();  // keep the host copy current (rr, r2max set on the device)
     }
 
-    // One iteration on the runner's stream (what the graphs hold), with the
-    // hopping term blocked nr right-hand sides per thread and every per-system
-    // grid sized for nr * groups live systems.
-    void iteration(int nr, int groups)
+    // One iteration on the runner's stream, with the hopping term blocked nr right-hand
+    // sides per thread and every per-system grid sized for nr * groups live systems. It reads
+    // the live list of phase ph and its k_cgm_scalars2 writes the list of phase 1 - ph for the
+    // next iteration; the list of the first iteration of a chunk comes from k_cgm_begin.
+    void iteration(int nr, int groups, int ph)
     {
-        const CgmLaunch L = launch(nr * groups);
+        const CgmLaunch L = launch(nr * groups, ph);
         CgmState<S> *st = state_.as<CgmState<S>>();
+#if !KQ_LIST_IN_SCALARS2
         cgm_begin<S>(st, L, st_);
+#endif
         DslashArgs<LF, LL, VR> d{};
         d.g = g_;
         d.fat = fat_;
         d.lng = lng_;
-        d.run = any();
-        d.rlist = rlist();
-        d.nact = nact();
+        d.run = L.any;
+        d.rlist = L.rlist;
+        d.nact = L.nact;
         d.nrhs = L.nlaunch;
         d.dst_parity = 1 - parity_;
         d.src = vec_p0();
@@ -629,9 +713,9 @@ private:
         cgm_xp_update<S, VR, VR>(st, x_.as<VR>(), p_.as<VR>(), vec(r_), L, st_);
     }
 
-    // graph of one iteration for nlive live systems: right-hand-side blocking nr
-    // from the tuning table and ceil(nlive / nr) groups (no grid slots for
-    // systems that cannot be live during the chunk)
+    // graph of two iterations (live lists of phase 0 then 1) for nlive live systems:
+    // right-hand-side blocking nr from the tuning table and ceil(nlive / nr) groups (no grid
+    // slots for systems that cannot be live during the chunk)
     hipGraphExec_t graph_for(int nlive)
     {
         const int nr = tune_.pick_nr(nlive);
@@ -640,7 +724,8 @@ private:
         if (!gx) {
             hipGraph_t graph;
             KQ_HIP_CHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
-            iteration(nr, groups);
+            iteration(nr, groups, 0);
+            iteration(nr, groups, 1);
             KQ_HIP_CHECK(hipStreamEndCapture(st_, &graph));
             KQ_HIP_CHECK(hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0));
             KQ_HIP_CHECK(hipGraphDestroy(graph));
@@ -673,6 +758,8 @@ private:
     LinkD fat64_{}, lng64_{};
     double delta_ = 0.1, pause2_ = 0.0;
     DevBuf b_hi_, x_hi_, tmp64_, ax64_;
+    int ndslot_ = 0, dslot_used_ = 0;
+    DevBuf shist_, rhist_, p0snap_, coef_;
     int reliable_updates_ = 0;
     int restarts_ = 0;
     std::map<std::pair<int, int>, hipGraphExec_t> graphs_;

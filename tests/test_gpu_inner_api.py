@@ -87,8 +87,10 @@ def test_inner_classes_against_oracle(tmp_path, kq, mode, lat):
     ref_mdm = oracle.mdaggm(fat, lng, b, lat, 0)
     assert rel_err(mdm, ref_mdm) <= op_tol
     b64, m64 = b.astype(np.float64), mdm.astype(np.float64)
-    assert abs(norm2 - np.sum(b64 * b64)) <= 1e-12 * np.sum(b64 * b64)
-    assert abs(dot - np.sum(b64 * m64)) <= 1e-12 * np.linalg.norm(b64) * np.linalg.norm(m64)
+    # the sums are formed in FP64 and returned as T (spinor_field.h:48-64)
+    red_tol = 1e-12 if mode == "double" else 1e-6
+    assert abs(norm2 - np.sum(b64 * b64)) <= red_tol * np.sum(b64 * b64)
+    assert abs(dot - np.sum(b64 * m64)) <= red_tol * np.linalg.norm(b64) * np.linalg.norm(m64)
     assert rel_err(axpby, 0.75 * b64 - 1.25 * m64) <= (1e-15 if mode == "double" else 1e-6)
 
     xo, ito = oracle.cgm(fat, lng, lat, 0, shifts.astype(dt), b, tol, max_iter)
