@@ -47,13 +47,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(path: Path):
-    """HBM bytes per launch of the FP64 hopping term from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+def pmc_traffic(path: Path, kernel: str = "k_dslash<kq::LinkD, kq::LinkD, double, 1"):
+    """HBM bytes per launch of a hopping-term kernel (default: the FP64 term) from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
     if not path.exists():
         return None
-    m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", path.read_text())
-    return float(m.group(1)) if m else None
+    for block in path.read_text().split("\nkernel: ")[1:]:
+        if block.startswith(kernel):
+            m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", block)
+            return float(m.group(1)) if m else None
+    return None
 
 
 def host_cpu() -> dict:
@@ -291,6 +294,7 @@ def main() -> int:
     st, cp = A.bench_copy(1 << 30, 20)
     r64 = rl["fp64"]
     traffic = pmc_traffic(ROOT / "profiles" / "dslash_32x32x32x32_pmc.txt")
+    traffic_x4 = pmc_traffic(ROOT / "profiles" / "dslash_32x32x32x32_pmc.txt", "k_dslash<kq::LinkF, kq::LinkF12, float, 4")
 
     # CPU baseline: the oracle CGM (the reference's algorithm, FP64) solving the same problem to
     # completion on the host cores; its solution also gives each shift's error
@@ -373,7 +377,8 @@ def main() -> int:
                               "bytes_per_site": round(rl["fp32_l16"].bytes / Vh, 1)},
         "roofline_fp32_x4": {"achieved": round(r4.gbs, 1), "frac": round(r4.hbm_frac, 4),
                              "us_per_launch": round(r4.ms * 1e3, 2), "bytes_per_site": round(r4.bytes / Vh, 1),
-                             "nr": r4.nr, "nt": r4.nt},
+                             "nr": r4.nr, "nt": r4.nt, "traffic": traffic_x4,
+                             "kernel": "k_dslash<LinkF,LinkF12,float,4> (the mixed solve's dominant kernel)"},
         "roofline_fp16": {"achieved": round(rl["fp16"].gbs, 1), "frac": round(rl["fp16"].hbm_frac, 4),
                           "us_per_launch": round(rl["fp16"].ms * 1e3, 2),
                           "bytes_per_site": round(rl["fp16"].bytes / Vh, 1)},

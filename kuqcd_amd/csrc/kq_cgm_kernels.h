@@ -58,14 +58,9 @@ struct CgmLaunch {
     uint32_t nblk_dsl;  // blocks of the hopping-term launch (= stride of part_a)
     int nsys;           // system slots (k_cgm_begin scans all of them)
     int nlaunch;        // slots of the live list the per-system kernels cover (>= live systems)
-    int *rlist;         // compact list of live systems this iteration (k_cgm_begin at the start of a
-                        // chunk, then the previous iteration's k_cgm_scalars2)
+    int *rlist;         // compact list of live systems (written by k_cgm_begin)
     int *nact;          // its length
     int *any;           // 1 while any system is live
-    int *rlist_next;    // the list of the next iteration, written by k_cgm_scalars2 (null: none)
-    int *nact_next;
-    int *any_next;
-    int *done;          // completion counter of k_cgm_scalars2's workgroups (0 between launches)
     double *part_a;     // [nsys][nblk_dsl] partial sums of <p, A p>
     double *part_b;     // [nsys][nblk] partial sums of |r|^2
     const size_t *off;  // [nsys] element offset of each system's p/x block

@@ -110,25 +110,15 @@ template <typename S> void cgm_scalars1(CgmState<S> *st, const CgmLaunch &L, hip
 // ------------------------------------------------------------ scalars 2
 // |r|^2 new, alpha recurrences, drop converged shifts from the end,
 // iteration count and stop flag (inverter_cgm.cpp:58-78); pause flag for the
-// reliable-update schedule. The last of its workgroups to finish writes the next
-// iteration's list of live systems (what k_cgm_begin does at the start of a chunk),
-// so an iteration needs no separate list kernel.
+// reliable-update schedule.
 template <typename S>
 __global__ void __launch_bounds__(256) k_cgm_scalars2(CgmState<S> *st, const int *rlist, const int *nact,
-                                                      const double *part, uint32_t nblk, double *shist, int nsys,
-                                                      int *rlist_next, int *nact_next, int *any_next, int *done)
+                                                      const double *part, uint32_t nblk, double *shist)
 {
 #pragma clang fp contract(off)
     const int slot = blockIdx.x;
-    const int n_live = *nact;
-    if (n_live <= slot) {
-        // nothing is live this iteration, and no system becomes live inside a chunk
-        if (n_live == 0 && slot == 0 && threadIdx.x == 0 && rlist_next != nullptr) {
-            *nact_next = 0;
-            *any_next = 0;
-        }
+    if (*nact <= slot)
         return;
-    }
     const int r = rlist[slot];
     const double l2_d = block_reduce_fixed(part + (size_t)r * nblk, nblk);
     CgmState<S> &c = st[r];
@@ -168,29 +158,12 @@ __global__ void __launch_bounds__(256) k_cgm_scalars2(CgmState<S> *st, const int
         c.stop = 1;
     if (c.pause2 > 0.0 && l2_d < c.pause2 * c.r2max)
         c.pause = 1;
-    if (rlist_next == nullptr)
-        return;
-    // the last workgroup (of the n_live that get here) sees every system's stop and pause flags
-    __threadfence();
-    if (atomicAdd(done, 1) != n_live - 1)
-        return;
-    __threadfence();
-    int n = 0;
-    for (int q = 0; q < nsys; ++q) {
-        const int live = (st[q].stop == 0) && (st[q].pause == 0);
-        st[q].run = live;
-        if (live)
-            rlist_next[n++] = q;
-    }
-    *nact_next = n;
-    *any_next = n > 0;
-    *done = 0;
 }
 
 template <typename S> void cgm_scalars2(CgmState<S> *st, const CgmLaunch &L, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_cgm_scalars2<S>, dim3(L.nlaunch), dim3(256), 0, stream, st, L.rlist, L.nact, L.part_b,
-                       L.nblk, L.shist, L.nsys, L.rlist_next, L.nact_next, L.any_next, L.done);
+                       L.nblk, L.shist);
     KQ_HIP_CHECK(hipGetLastError());
 }
 

@@ -29,10 +29,6 @@
 #include "kq_dslash.h"
 #include "kq_layout.h"
 
-#ifndef KQ_LIST_IN_SCALARS2
-#define KQ_LIST_IN_SCALARS2 1
-#endif
-
 namespace kq {
 
 // Device-memory cache: released buffers are kept by size and handed out again to
@@ -253,18 +249,14 @@ public:
         // launch_dslash accepts fit either
         part_a_.alloc(sizeof(double) * max_sys * dslash_nblk(g.Vh, kMinDslashNT));
         part_b_.alloc(sizeof(double) * max_sys * nblk_);
-        // two live lists (iterations alternate between them), the staged system list, and the
-        // completion counter of k_cgm_scalars2
-        ints_.alloc(sizeof(int) * (3 * max_sys + 8));
-        KQ_HIP_CHECK(hipMemsetAsync(ints_.as<void>(), 0, ints_.bytes(), st_));
-        if (tune_.chunk % 2 != 0)
-            tune_.chunk += 1;  // iterations run in pairs, one per live list
+        ints_.alloc(sizeof(int) * (2 * max_sys + 4));
         sigma_.alloc(sizeof(double) * max_sys);
         off_.alloc(sizeof(size_t) * max_sys);
         off_h_.assign(max_sys, 0);
         sig_h_.assign(max_sys, 0.0);
         last_fold_.assign(max_sys, 0);
         prev_max_.assign(max_sys, 0);
+        rel_rr_.assign(max_sys, {});
         finalized_.assign(max_sys, 1);
         KQ_HIP_CHECK(hipMemcpyAsync(state_.as<void>(), state_h_.get(), sizeof(CgmState<S>) * max_sys,
                                     hipMemcpyHostToDevice, st_));
@@ -380,17 +372,13 @@ public:
             int nlive = 0;
             for (int sys = 0; sys < nsys_; ++sys)
                 nlive += (!state_h_[sys].stop && !state_h_[sys].pause) ? 1 : 0;
-            // the chunk's first live list comes from k_cgm_begin; each iteration's
-            // k_cgm_scalars2 writes the next one
             if (direct_launches()) {
                 const int nr = tune_.pick_nr(nlive);
-                cgm_begin<S>(state_.as<CgmState<S>>(), launch(1, 0), st_);
                 for (int c = 0; c < tune_.chunk; ++c)
-                    iteration(nr, std::max(1, (nlive + nr - 1) / nr), c & 1);
+                    iteration(nr, std::max(1, (nlive + nr - 1) / nr));
             } else {
-                hipGraphExec_t gx = graph_for(nlive);  // captured before anything is queued
-                cgm_begin<S>(state_.as<CgmState<S>>(), launch(1, 0), st_);
-                for (int c = 0; c < tune_.chunk; c += 2)
+                hipGraphExec_t gx = graph_for(nlive);
+                for (int c = 0; c < tune_.chunk; ++c)
                     KQ_HIP_CHECK(hipGraphLaunch(gx, st_));
             }
             const auto tb = clk::now();
@@ -445,11 +433,28 @@ public:
                 reliable_update(fold, rel);
                 t_rel += std::chrono::duration<double>(clk::now() - tr).count();
             }
+            // A single-shift system whose true residual has fallen by less than a decade over the
+            // last kStallUpdates reliable updates has stalled on the sloppy operator (a healthy one
+            // gains a decade or more per update; seen with 16-bit fat links on ill-conditioned
+            // systems, where the base system crept to 1e-10 over ~39 000 iterations): it stops, and
+            // the caller's FP64 verification refines it from a fresh start.
+            std::vector<int> stalled;
+            for (int sys : rel) {
+                const CgmState<S> &c = state_h_[sys];
+                std::vector<double> &h = rel_rr_[sys];
+                h.push_back(c.r2max);
+                if (h.size() > (size_t)kStallUpdates + 1)
+                    h.erase(h.begin());
+                if (h.size() == (size_t)kStallUpdates + 1 && !(h.back() < 1e-2 * h.front()) && !c.stop &&
+                    c.maxTerm <= 1)
+                    stalled.push_back(sys);
+            }
             // restart single-shift systems whose iterated residual was far below the true one
             std::vector<int> restart;
             for (int sys : rel) {
                 const CgmState<S> &c = state_h_[sys];
-                if (!c.stop && c.maxTerm <= 1 && !((double)c.rr < 1e2 * rr_iter[sys]))
+                if (!c.stop && c.maxTerm <= 1 && !((double)c.rr < 1e2 * rr_iter[sys]) &&
+                    std::find(stalled.begin(), stalled.end(), sys) == stalled.end())
                     restart.push_back(sys);
             }
             if (!restart.empty()) {
@@ -464,6 +469,14 @@ public:
                 download_states();
                 if (dbg)
                     fprintf(stderr, "[kq] restarted %zu single-shift systems\n", restart.size());
+            }
+            if (!stalled.empty()) {
+                for (int sys : stalled)
+                    state_h_[sys].stop = 1;
+                upload_states();
+                stalls_ += (int)stalled.size();
+                if (dbg)
+                    fprintf(stderr, "[kq] stopped %zu stalled single-shift systems\n", stalled.size());
             }
             if (on_drop)
                 for (const auto &d : drops)
@@ -489,20 +502,19 @@ public:
     int nsys() const { return nsys_; }
     int reliable_updates() const { return reliable_updates_; }
     int restarts() const { return restarts_; }
+    int stalls() const { return stalls_; }
     const Geom &geom() const { return g_; }
 
 private:
     Vec<VR> vec(const DevBuf &b) const { return Vec<VR>{b.as<V2>(), g_.Vh, n3_}; }
     Vec<VR> vec_p0() const { return Vec<VR>{p_.as<V2>(), g_.Vh, 0, off_.as<size_t>()}; }
-    // live list of iterations with phase 0 (the first of a chunk) and phase 1
-    int *rlist(int ph = 0) const { return ints_.as<int>() + (ph ? 2 * max_sys_ + 3 : 0); }
-    int *nact(int ph = 0) const { return ints_.as<int>() + (ph ? 3 * max_sys_ + 3 : max_sys_); }
-    int *any(int ph = 0) const { return ints_.as<int>() + (ph ? 3 * max_sys_ + 4 : max_sys_ + 1); }
+    int *rlist() const { return ints_.as<int>(); }
+    int *nact() const { return ints_.as<int>() + max_sys_; }
+    int *any() const { return ints_.as<int>() + max_sys_ + 1; }
     int *list() const { return ints_.as<int>() + max_sys_ + 2; }
     int *list_n() const { return ints_.as<int>() + 2 * max_sys_ + 2; }
-    int *done_counter() const { return ints_.as<int>() + 3 * max_sys_ + 5; }
 
-    CgmLaunch launch(int nlaunch, int ph = 0) const
+    CgmLaunch launch(int nlaunch) const
     {
         CgmLaunch L;
         L.Vh = g_.Vh;
@@ -510,21 +522,9 @@ private:
         L.nblk_dsl = nblk_dsl_;
         L.nsys = max_sys_;
         L.nlaunch = std::min(std::max(nlaunch, 1), max_sys_);
-#if KQ_LIST_IN_SCALARS2
-        L.rlist = rlist(ph);
-        L.nact = nact(ph);
-        L.any = any(ph);
-        L.rlist_next = rlist(1 - ph);
-        L.nact_next = nact(1 - ph);
-        L.any_next = any(1 - ph);
-#else
-        (void)ph;
-        L.rlist = rlist(0);
-        L.nact = nact(0);
-        L.any = any(0);
-        L.rlist_next = L.nact_next = L.any_next = nullptr;
-#endif
-        L.done = done_counter();
+        L.rlist = rlist();
+        L.nact = nact();
+        L.any = any();
         L.part_a = part_a_.as<double>();
         L.part_b = part_b_.as<double>();
         L.off = off_.as<size_t>();
@@ -606,6 +606,7 @@ private:
             last_fold_[sys] = state_h_[sys].nshift;
             prev_max_[sys] = state_h_[sys].nshift;
             finalized_[sys] = 0;
+            rel_rr_[sys].clear();
         }
         upload_states();
         KQ_HIP_CHECK(hipMemcpyAsync(off_.as<void>(), off_h_.data(), sizeof(size_t) * max_sys_, hipMemcpyHostToDevice,
@@ -677,17 +678,14 @@ private:
         download_states();  // keep the host copy current (rr, r2max set on the device)
     }
 
-    // One iteration on the runner's stream, with the hopping term blocked nr right-hand
-    // sides per thread and every per-system grid sized for nr * groups live systems. It reads
-    // the live list of phase ph and its k_cgm_scalars2 writes the list of phase 1 - ph for the
-    // next iteration; the list of the first iteration of a chunk comes from k_cgm_begin.
-    void iteration(int nr, int groups, int ph)
+    // One iteration on the runner's stream (what the graphs hold), with the
+    // hopping term blocked nr right-hand sides per thread and every per-system
+    // grid sized for nr * groups live systems.
+    void iteration(int nr, int groups)
     {
-        const CgmLaunch L = launch(nr * groups, ph);
+        const CgmLaunch L = launch(nr * groups);
         CgmState<S> *st = state_.as<CgmState<S>>();
-#if !KQ_LIST_IN_SCALARS2
         cgm_begin<S>(st, L, st_);
-#endif
         DslashArgs<LF, LL, VR> d{};
         d.g = g_;
         d.fat = fat_;
@@ -713,9 +711,9 @@ private:
         cgm_xp_update<S, VR, VR>(st, x_.as<VR>(), p_.as<VR>(), vec(r_), L, st_);
     }
 
-    // graph of two iterations (live lists of phase 0 then 1) for nlive live systems:
-    // right-hand-side blocking nr from the tuning table and ceil(nlive / nr) groups (no grid
-    // slots for systems that cannot be live during the chunk)
+    // graph of one iteration for nlive live systems: right-hand-side blocking nr
+    // from the tuning table and ceil(nlive / nr) groups (no grid slots for
+    // systems that cannot be live during the chunk)
     hipGraphExec_t graph_for(int nlive)
     {
         const int nr = tune_.pick_nr(nlive);
@@ -724,8 +722,7 @@ private:
         if (!gx) {
             hipGraph_t graph;
             KQ_HIP_CHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
-            iteration(nr, groups, 0);
-            iteration(nr, groups, 1);
+            iteration(nr, groups);
             KQ_HIP_CHECK(hipStreamEndCapture(st_, &graph));
             KQ_HIP_CHECK(hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0));
             KQ_HIP_CHECK(hipGraphDestroy(graph));
@@ -762,6 +759,9 @@ private:
     DevBuf shist_, rhist_, p0snap_, coef_;
     int reliable_updates_ = 0;
     int restarts_ = 0;
+    int stalls_ = 0;
+    static constexpr int kStallUpdates = 5;
+    std::vector<std::vector<double>> rel_rr_;  // true |r|^2 at the last kStallUpdates + 1 reliable updates
     std::map<std::pair<int, int>, hipGraphExec_t> graphs_;
 };
 
