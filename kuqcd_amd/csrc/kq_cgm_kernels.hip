@@ -520,15 +520,20 @@ void seg_coef(const CgmState<S> *st, const double *shist, double *coef, const in
     KQ_HIP_CHECK(hipGetLastError());
 }
 
-// One thread per complex entry of a vector; blockIdx.x = shift (fastest, so the blocks of all
-// shifts of one site range run together and read the recorded residuals from L2).
+// One thread per complex entry of a vector. Workgroups are dealt to the 8 XCDs round robin
+// (linear id % 8), so blockIdx.x = 8 (k + kmax * (y / 8)) + y % 8 for shift k and site range y:
+// the kmax workgroups of one site range run back to back on one XCD and read its recorded
+// residuals once into that XCD's L2.
 template <typename S, typename VR>
 __global__ void __launch_bounds__(kBlasNT) k_seg_fold(const CgmState<S> *st, const double *coef, double *x_hi, VR *p,
                                                       const VR *p0snap, const VR *rhist, const size_t *off,
-                                                      uint32_t Vh, const int *list, const int *n)
+                                                      uint32_t Vh, const int *list, const int *n, int kmax)
 {
     using V2 = c2_t<VR>;
-    const int k = blockIdx.x, slot = blockIdx.z;
+    const int slot = blockIdx.z;
+    const uint32_t rest = blockIdx.x / 8;
+    const int k = (int)(rest % (uint32_t)kmax);
+    const size_t y = (size_t)(rest / (uint32_t)kmax) * 8 + blockIdx.x % 8;
     if (*n <= slot)
         return;
     const int rh = list[slot];
@@ -536,7 +541,7 @@ __global__ void __launch_bounds__(kBlasNT) k_seg_fold(const CgmState<S> *st, con
     if (c.dslot < 0 || k >= c.maxTermFold)
         return;
     const size_t n3 = (size_t)3 * Vh;
-    const size_t e = (size_t)blockIdx.y * kBlasNT + threadIdx.x;
+    const size_t e = y * kBlasNT + threadIdx.x;
     if (e >= n3)
         return;
     const bool keep_x = fold_keeps(c.discard, k, c.maxTerm);
@@ -584,8 +589,12 @@ void seg_fold(const CgmState<S> *st, const double *coef, double *x_hi, VR *p, co
     if (kmax <= 0)
         return;
     const size_t n3 = (size_t)3 * Vh;
-    hipLaunchKernelGGL((k_seg_fold<S, VR>), dim3(kmax, (unsigned)((n3 + kBlasNT - 1) / kBlasNT), nsys), dim3(kBlasNT),
-                       0, stream, st, coef, x_hi, p, p0snap, rhist, off, Vh, list, n);
+    const size_t nb8 = (n3 + 8 * kBlasNT - 1) / (8 * kBlasNT);  // site ranges per XCD
+    const size_t gx = 8 * (size_t)kmax * nb8;
+    if (kmax > kMaxShift || gx > 0x7fffffffu)
+        throw std::invalid_argument("seg_fold: grid too large");
+    hipLaunchKernelGGL((k_seg_fold<S, VR>), dim3((unsigned)gx, 1, nsys), dim3(kBlasNT), 0, stream, st, coef, x_hi, p,
+                       p0snap, rhist, off, Vh, list, n, kmax);
     KQ_HIP_CHECK(hipGetLastError());
 }
 

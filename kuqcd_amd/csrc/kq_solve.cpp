@@ -73,6 +73,10 @@ static KuQCDStatus solve_ref_t(int slot, LK fat, LK lng, void **x, void *b, KuQC
     set_caps(tune, slot, KQ_MODE_REFERENCE, false);
     tune.chunk = 16;
     CgmRunner<T, T, LK, LK> run(g, parity, fat, lng, 1, ns, c.stream, tune);
+    // FP64 solutions take their updates per segment of recorded residuals: the same iterations
+    // and scalars as updating every shift each iteration, with a quarter of the vector traffic
+    if constexpr (std::is_same<T, double>::value)
+        run.enable_deferred(deferred_slots<double>(1, g.Vh));
     DevBuf stage, bdev(sizeof(c2_t<T>) * 3 * (size_t)g.Vh);
     const size_t n3 = (size_t)3 * g.Vh;
     upload_vec<T, T>(b, Vec<T>{bdev.as<c2_t<T>>(), g.Vh, n3}, 0, stage, c.stream);
@@ -164,12 +168,6 @@ static bool debug_on()
 {
     static const bool on = getenv("KQ_DEBUG") != nullptr;
     return on;
-}
-
-static bool deferred_updates()
-{
-    const char *e = getenv("KQ_DEFERRED");
-    return !(e != nullptr && e[0] == '0');
 }
 
 // autotuned right-hand-side blocking for 0..min(max_sys, 16) live systems
@@ -271,8 +269,7 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
     run.enable_reliable(link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.delta);
     // the multishift systems fold their solution updates per segment instead of updating every
     // shift's x and p each iteration (KQ_DEFERRED=0 switches this off)
-    if (deferred_updates())
-        run.enable_deferred(nrhs);
+    run.enable_deferred(deferred_slots<float>(nrhs, P.g.Vh));
     std::vector<int> sys_of(nrhs);
     std::vector<int> rhs_of(max_sys, -1);
     std::vector<int> forked((size_t)nrhs * ns, -1);
@@ -409,6 +406,7 @@ static void run_fp64(Problem &P, DevBuf &x_out, KqSolveReport *rep, std::vector<
     auto t0 = clk::now();
     CgmRunner<double, double, LinkD, LinkD> run(P.g, P.parity, link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.nrhs,
                                                 (size_t)P.nrhs * P.ns, c.stream, tune);
+    run.enable_deferred(deferred_slots<double>(P.nrhs, P.g.Vh));
     for (int r = 0; r < P.nrhs; ++r) {
         std::vector<double> tolk(P.ns, P.tol[r]);
         run.add_system(P.ns, P.p->shifts + (size_t)r * P.ns, tolk.data(), P.p->max_iter,

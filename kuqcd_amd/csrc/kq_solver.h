@@ -23,6 +23,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <type_traits>
 #include <vector>
 
 #include "kq_cgm_kernels.h"
@@ -103,6 +104,29 @@ inline bool direct_launches()
         return e != nullptr && e[0] == '1';
     }();
     return on;
+}
+
+// KQ_DEFERRED=0 makes the multishift systems update every shift's x and p each iteration
+// instead of folding the updates per segment of recorded residuals
+inline bool deferred_updates_on()
+{
+    static const bool on = [] {
+        const char *e = getenv("KQ_DEFERRED");
+        return !(e != nullptr && e[0] == '0');
+    }();
+    return on;
+}
+
+// History slots for the deferred updates of up to nsys multishift systems: each slot records
+// kHistMax residuals of VR precision; the slots stay within 32 GiB (at 48^3x96 in FP64, four
+// slots), and systems beyond them update every iteration
+template <typename VR> inline int deferred_slots(int nsys, uint32_t Vh)
+{
+    if (!deferred_updates_on() || nsys <= 0)
+        return 0;
+    const size_t per = sizeof(c2_t<VR>) * 3 * (size_t)Vh * kHistMax;
+    const size_t budget = (size_t)32 << 30;
+    return (int)std::min<size_t>((size_t)nsys, std::max<size_t>(1, budget / per));
 }
 
 class DevBuf {
@@ -287,13 +311,14 @@ public:
         ax64_.alloc(sizeof(double2) * max_sys_ * n3_);
     }
 
-    // Deferred solution updates (reliable mode) for up to `slots` multishift systems: each
-    // records its residuals and recurrence scalars, keeps only p_0 current, and has its x_k
-    // and p_k brought up to date by segment folds (kq_cgm_kernels.h)
+    // Deferred solution updates for up to `slots` multishift systems: each records its residuals
+    // and recurrence scalars, keeps only p_0 current, and has its x_k and p_k brought up to date
+    // by segment folds (kq_cgm_kernels.h). The folds accumulate in FP64: into x_hi in the
+    // reliable mode, into the solutions themselves when they are FP64 (VR = double).
     void enable_deferred(int slots)
     {
-        if (!reliable_)
-            throw std::logic_error("deferred solution updates need the reliable mode");
+        if (!reliable_ && !std::is_same<VR, double>::value)
+            throw std::logic_error("deferred solution updates need FP64 solutions or the reliable mode");
         if (tune_.chunk > kHistMax)
             throw std::invalid_argument("deferred solution updates: chunk longer than the history");
         ndslot_ = std::max(0, slots);
@@ -403,7 +428,7 @@ public:
                 prev_max_[sys] = std::max(0, c.maxTerm);
                 if (c.stop) {
                     finalized_[sys] = 1;
-                    if (reliable_) {
+                    if (reliable_ || c.dslot >= 0) {
                         c.maxTermFold = last_fold_[sys];
                         fold.push_back(sys);
                     }
@@ -419,7 +444,7 @@ public:
                 c.discard = diverging ? 1 : 0;  // the updates since the last fold are not trusted
                 // a recorded segment is folded before the next chunk could overrun the history
                 const bool hist_full = c.dslot >= 0 && c.hist + tune_.chunk > kHistMax;
-                if (reliable_ && (need_rel || dropped || hist_full)) {
+                if ((reliable_ || c.dslot >= 0) && (need_rel || dropped || hist_full)) {
                     c.maxTermFold = last_fold_[sys];
                     fold.push_back(sys);
                 }
@@ -646,7 +671,7 @@ private:
                 if (state_h_[sys].dslot >= 0) {
                     seg.push_back(sys);
                     kmax = std::max(kmax, state_h_[sys].maxTermFold);
-                } else {
+                } else if (reliable_) {
                     plain.push_back(sys);
                 }
             }
@@ -659,8 +684,10 @@ private:
                 set_list(seg);
                 CgmState<S> *stp = state_.as<CgmState<S>>();
                 seg_coef<S>(stp, shist_.as<double>(), coef_.as<double>(), list(), list_n(), list_grid(), st_);
-                seg_fold<S, VR>(stp, coef_.as<double>(), x_hi_.as<double>(), p_.as<VR>(), p0snap_.as<VR>(),
-                                rhist_.as<VR>(), off_.as<size_t>(), g_.Vh, list(), list_n(), list_grid(), kmax, st_);
+                // the folds accumulate into the FP64 solutions: x_hi, or x_lo itself when VR is double
+                double *xf = reliable_ ? x_hi_.as<double>() : x_.as<double>();
+                seg_fold<S, VR>(stp, coef_.as<double>(), xf, p_.as<VR>(), p0snap_.as<VR>(), rhist_.as<VR>(),
+                                off_.as<size_t>(), g_.Vh, list(), list_n(), list_grid(), kmax, st_);
                 seg_reset<S, VR>(stp, p_.as<VR>(), p0snap_.as<VR>(), off_.as<size_t>(), g_.Vh, list(), list_n(),
                                  list_grid(), st_);
             }
