@@ -5,7 +5,9 @@
 // behaviour the library returns an error instead:
 //   halo_depth not in {0,3}, gauge site_subset != ALL, odd Lx  -> INVALID_PARAM (kuqcd.cpp:62-99, 342)
 //   param->prec != size of the loaded gauge's real            -> INVALID_PARAM (lattice_field.cpp:31)
-//   KUQCD_HISQ_DIRAC (non even-odd) solve                     -> UNSUPPORTED   (dirac_hisq.cpp:130, 422)
+//   KUQCD_HISQ_DIRAC (non even-odd) solve: the reference gives it half-lattice fields that its
+//     operator indexes as full ones (dirac_hisq.cpp:130, 422). kq_solve_multishift solves on
+//     fields of both parities (V sites, even block first); the extensions -> UNSUPPORTED
 //   more than 32 shifts                                       -> UNSUPPORTED
 #include <chrono>
 #include <cmath>
@@ -319,8 +321,9 @@ static KuQCDStatus load_gauge(void *h_gauge, KuQCDGaugeParam *param)
 
 // ---------------------------------------------------------------- solve
 // Checks of kuqcd_solve_multishift + kuqcd_inverter (kuqcd.cpp:407-414, 176-199, 214)
-// and of the extensions; on success *slot is the gauge pair to use.
-KuQCDStatus check_solve(void **x, void *b, KuQCDInvertParam *param, int nrhs, int *slot)
+// and of the extensions; on success *slot is the gauge pair to use. full_ok: the caller
+// solves KUQCD_HISQ_DIRAC on fields of both parities.
+KuQCDStatus check_solve(void **x, void *b, KuQCDInvertParam *param, int nrhs, int *slot, bool full_ok = false)
 {
     if (x == nullptr || b == nullptr || param == nullptr) {
         set_error("hp_x or hp_b or param is nullptr");
@@ -364,8 +367,8 @@ KuQCDStatus check_solve(void **x, void *b, KuQCDInvertParam *param, int nrhs, in
         set_error("param->prec must equal the precision of the loaded gauge field");
         return KUQCD_ERR_INVALID_PARAM;
     }
-    if (param->dslash_type == KUQCD_HISQ_DIRAC) {
-        set_error("only the even-odd preconditioned operator (KUQCD_HISQPC_DIRAC) is supported");
+    if (param->dslash_type == KUQCD_HISQ_DIRAC && !full_ok) {
+        set_error("this entry point solves the even-odd preconditioned operator (KUQCD_HISQPC_DIRAC) only");
         return KUQCD_ERR_UNSUPPORTED;
     }
     if (param->num_shift > kMaxShift) {
@@ -450,7 +453,7 @@ KuQCDStatus kq_solve_multishift(void **x, void *b, KuQCDInvertParam *param)
 {
     return guarded([&] {
         int slot = -1;
-        KuQCDStatus st = check_solve(x, b, param, 1, &slot);
+        KuQCDStatus st = check_solve(x, b, param, 1, &slot, true);
         if (st != KUQCD_SUCCESS)
             return st;
         return solve_reference(slot, x, b, *param);

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "kq_blas.h"
+#include "kq_cgm_host.h"
 #include "kq_context.h"
 #include "kq_solve.h"
 
@@ -404,53 +405,31 @@ void print_iters(int iter, int max_iter)
 }
 
 // The recurrence of inverter_cgm.cpp:17-87 driven from the host over any Dirac operator
-// (a caller's own Dirac subclass): per-shift scalars alpha, beta, zeta, zeta_prev in T.
+// (a caller's own Dirac subclass), with the inner API's fields and BLAS-1
+template <typename T, typename VecT> struct FieldOps {
+    Dirac<VecT> *dirac;
+    std::vector<VecT> p;
+    VecT ap, r;
+    VecT **sol;
+    T norm2_r() { return Norm2(r); }
+    T apply(T sigma0)
+    {
+        dirac->MdaggM(p[0], ap, true);
+        Axmy(ap, p[0], sigma0);  // ap = sigma_0 p0 - D D p0
+        return RealDot(p[0], ap);
+    }
+    void update_r(T b0) { Axpy(r, ap, b0); }
+    void update_x(int k, T bk) { Ymax(*sol[k], p[k], bk); }
+    void update_p(int k, T zk, T ak) { Axpby(p[k], r, zk, ak); }
+};
+
 template <typename T, typename VecT>
 int cgm_host(Dirac<VecT> *dirac, int ns, const T *shift, const VecT &src, VecT *sol[], double tol, int max_iter)
 {
-    struct Shift {
-        T alpha = 0, beta = 1, zeta = 1, zeta_prev = 1;
-    };
-    std::vector<Shift> sh(ns);
-    std::vector<VecT> p(ns, src);  // p_k = src
-    VecT ap(src.latBase), r(src), p0(src.latBase);
+    FieldOps<T, VecT> ops{dirac, std::vector<VecT>(ns, src), VecT(src.latBase), VecT(src), sol};
     for (int k = 0; k < ns; ++k)
         sol[k]->Clear();
-    T rr = Norm2(r), beta_prev = 1.0;
-    int live = ns, iter = 0;
-    do {
-        ++iter;
-        p0 = p[0];
-        dirac->MdaggM(p0, ap, true);
-        Axmy(ap, p0, shift[0]);  // ap = sigma_0 p0 - D D p0
-        const T pap = RealDot(p0, ap);
-        Shift &s0 = sh[0];
-        s0.beta = -rr / pap;
-        Axpy(r, ap, s0.beta);
-        for (int k = 1; k < live; ++k) {
-            Shift &s = sh[k];
-            // the double literal keeps the reference's promotion for T = float
-            const T ratio = beta_prev * s.zeta_prev /
-                            (s0.beta * s0.alpha * (s.zeta_prev - s.zeta) + s.zeta_prev * beta_prev * (1.0 - shift[k] * s0.beta));
-            s.zeta_prev = s.zeta;
-            s.zeta = s.zeta * ratio;
-            s.beta = s0.beta * ratio;
-        }
-        beta_prev = s0.beta;
-        const T rr_new = Norm2(r);
-        s0.alpha = rr_new / rr;
-        rr = rr_new;
-        for (int k = 0; k < live; ++k)
-            Ymax(*sol[k], p[k], sh[k].beta);
-        for (int k = 1; k < live; ++k)
-            sh[k].alpha = s0.alpha * sh[k].zeta * sh[k].beta / (sh[k].zeta_prev * s0.beta);
-        for (int k = 0; k < live; ++k)
-            Axpby(p[k], r, sh[k].zeta, sh[k].alpha);
-        // drop converged shifts from the end (inverter_cgm.cpp:71-76)
-        while (live > 0 && sh[live - 1].zeta * sh[live - 1].zeta * rr_new < tol)
-            --live;
-    } while (live > 0 && iter < max_iter);
-    return iter;
+    return kq::cgm_recurrence<T>(ops, ns, shift, tol, max_iter);
 }
 
 }  // namespace

@@ -53,9 +53,10 @@ struct Context {
     std::string last_error;
     // autotuner cache: (mode, nrhs, Vh) -> (nr, nt)
     std::map<std::tuple<int, int, uint32_t>, std::pair<int, int>> tune;
-    // (mode, Vh) -> milliseconds of one group of NR = 1, 2, 4, 8 right-hand sides at block
-    // size 64, 128, 256 (index 3 * log2(NR) + log2(nt / 64)); < 0 where not measured
-    std::map<std::pair<int, uint32_t>, std::array<double, 12>> tune_group;
+    // (mode, Vh) -> milliseconds of one group of NR = 1, 2, 3, 4, 8 right-hand sides at block
+    // size 64, 128, 256 (index 3 * (position of NR in that list) + log2(nt / 64)); < 0 where
+    // not measured
+    std::map<std::pair<int, uint32_t>, std::array<double, 15>> tune_group;
     // (mode, Vh, nr, nt) -> occupancy cap of the hopping term (waves per SIMD, 0 = none)
     std::map<std::tuple<int, uint32_t, int, int>, int> tune_cap;
 };

@@ -8,6 +8,7 @@
 #include <hip/hip_fp16.h>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -75,12 +76,30 @@ struct Geom {
     uint32_t Vh;
     uint32_t stride[4];  // checkerboard index stride of y, z, t (stride[0] unused)
     FastDiv dx, dy, dz;  // divisions by Lx/2, Ly, Lz
-    // lattice tiles of the hopping term's block order (see tile_site): valid when
-    // Lz % 4 == 0 and Lt % 2 == 0; plane = (Lx/2)*Ly sites, zc = Lz/4, tc = Lt/2
+    // lattice tiles of the hopping term's block order (see tile_site): the 8 XCDs split y into
+    // 2^lyp ranges of yc rows, z into 2^lzp ranges of zc planes and t into the rest, ranges of tc;
+    // slab = (Lx/2)*yc sites of one (x, y) range, plane = (Lx/2)*Ly
     int tile;
-    uint32_t plane, zc, tc;
-    FastDiv dplane, dzc;
+    uint32_t plane, slab, yc, zc, tc, lyp, lzp;
+    FastDiv dslab, dzc;
 };
+
+// Largest (x, y) slab of a tile, in sites of one parity: the 32^4 plane (512 sites) keeps the
+// gathered vectors of the planes an XCD has in flight within its L2; the 1152-site planes of
+// 48^3x96 did not (FP64 term at 74 % of HBM peak there against 83 % at 32^4), so such planes
+// are split along y between XCDs.
+constexpr uint32_t kTileSlab = 640;
+// KQ_TILE_SLAB=<sites> replaces kTileSlab (a tuning switch for the hopping terms' site order;
+// results are unchanged except for the grouping of FP64 partial sums)
+inline uint32_t tile_slab_limit()
+{
+    static const uint32_t v = [] {
+        const char *e = getenv("KQ_TILE_SLAB");
+        const long n = e ? atol(e) : 0;
+        return n > 0 ? (uint32_t)n : kTileSlab;
+    }();
+    return v;
+}
 
 __host__ inline Geom make_geom(const int L[4])
 {
@@ -96,11 +115,32 @@ __host__ inline Geom make_geom(const int L[4])
     g.dx = FastDiv((uint32_t)g.lxh);
     g.dy = FastDiv((uint32_t)L[1]);
     g.dz = FastDiv((uint32_t)L[2]);
-    g.tile = (L[2] % 4 == 0 && L[3] % 2 == 0) ? 1 : 0;
+    // XCD splits (y ranges, z ranges, t ranges), 8 blocks each: the first that divides the
+    // lattice with a slab of at most kTileSlab sites, else the first that divides it
+    static const int kSplit[3][3] = {{1, 4, 2}, {2, 4, 1}, {4, 2, 1}};
+    int pick = -1;
+    for (int i = 0; i < 3; ++i) {
+        const int yp = kSplit[i][0], zp = kSplit[i][1], tp = kSplit[i][2];
+        if (L[1] % yp != 0 || L[2] % zp != 0 || L[3] % tp != 0)
+            continue;
+        const bool fits = (uint64_t)g.lxh * (L[1] / yp) <= tile_slab_limit();
+        if (pick < 0 || fits) {
+            pick = i;
+            if (fits)
+                break;
+        }
+    }
+    g.tile = pick >= 0 ? 1 : 0;
+    const int yp = pick >= 0 ? kSplit[pick][0] : 1, zp = pick >= 0 ? kSplit[pick][1] : 4;
+    const int tp = pick >= 0 ? kSplit[pick][2] : 2;
+    g.lyp = yp == 4 ? 2 : yp == 2 ? 1 : 0;
+    g.lzp = zp == 4 ? 2 : 1;
     g.plane = g.stride[2];
-    g.zc = (uint32_t)(L[2] / 4);
-    g.tc = (uint32_t)(L[3] / 2);
-    g.dplane = FastDiv(g.plane);
+    g.yc = (uint32_t)(L[1] / yp);
+    g.zc = (uint32_t)(L[2] / zp);
+    g.tc = (uint32_t)(L[3] / tp);
+    g.slab = (uint32_t)g.lxh * g.yc;
+    g.dslab = FastDiv(g.slab > 0 ? g.slab : 1);
     g.dzc = FastDiv(g.zc > 0 ? g.zc : 1);
     return g;
 }
@@ -193,22 +233,24 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb)
 
 // Site of thread tid of block b (of nb blocks of NT threads) for the stencil kernels.
 // Workgroups are dealt to the 8 XCDs round robin (b % 8), each XCD with its own L2.
-// When Vh is a multiple of 8*NT and the lattice splits into 4 z-ranges x 2 t-ranges,
-// XCD x owns the tile z in [(x%4) Lz/4, ...), t in [(x/4) Lt/2, ...) and walks it
-// plane by plane ((x,y) fastest, then z, then t), so the +-1 and +-3 neighbours along
-// z and t that a site gathers were mostly just read by the same XCD. Otherwise each
-// XCD gets a contiguous range of blocks (xcd_remap). Either way a bijection onto the
-// sites (tile: v < Vh/8 = plane*zc*tc maps to distinct (p, zl, tl) per XCD).
+// When Vh is a multiple of 8*NT and the lattice splits into 8 tiles (make_geom: y ranges x
+// z ranges x t ranges; 1 x 4 x 2 at 32^4, 2 x 4 x 1 at 48^3x96), XCD x owns one tile and walks
+// it slab by slab ((x, y) within its y range fastest, then z, then t), so the +-1 and +-3
+// neighbours along z and t that a site gathers were mostly just read by the same XCD.
+// Otherwise each XCD gets a contiguous range of blocks (xcd_remap). Either way a bijection onto
+// the sites (tile: v < Vh/8 = slab*zc*tc maps to distinct (p, zl, tl) per XCD).
 template <int NT> __device__ inline uint32_t tile_site(const Geom &g, uint32_t b, uint32_t nb, uint32_t tid, bool &valid)
 {
     if (g.tile && (g.Vh % (8u * NT)) == 0) {
         const uint32_t x = b & 7u, v = (b >> 3) * NT + tid;
-        const uint32_t q = g.dplane.div(v);
-        const uint32_t p = v - q * g.plane;
+        const uint32_t q = g.dslab.div(v);
+        const uint32_t p = v - q * g.slab;
         const uint32_t tl = g.dzc.div(q);
         const uint32_t zl = q - tl * g.zc;
+        const uint32_t xy = x & ((1u << g.lyp) - 1u), xz = (x >> g.lyp) & ((1u << g.lzp) - 1u);
+        const uint32_t xt = x >> (g.lyp + g.lzp);
         valid = true;
-        return p + g.plane * (((x & 3u) * g.zc + zl) + (uint32_t)g.L[2] * ((x >> 2) * g.tc + tl));
+        return p + g.slab * xy + g.plane * ((xz * g.zc + zl) + (uint32_t)g.L[2] * (xt * g.tc + tl));
     }
     const uint32_t s0 = xcd_remap(b, nb) * NT + tid;
     valid = s0 < g.Vh;

@@ -37,7 +37,7 @@ constexpr int kMinDslashNT = 64;
 constexpr int kReproNT = 256;
 
 // Launch D over all sites of parity dst_parity for every right-hand side.
-// nr = right-hand sides handled per thread (1, 2, 4 or 8), nt = threads per block,
+// nr = right-hand sides handled per thread (1, 2, 3, 4 or 8), nt = threads per block,
 // cap = at most this many wavefronts per SIMD (0: as many as the registers allow). The
 // cap is imposed with unused dynamic LDS; fewer resident waves keep the gathered
 // neighbour vectors of the batched terms in L2 (measured up to 19 % faster at 32^4).

@@ -232,6 +232,9 @@ void launch_dslash(DslashArgs<LF, LL, SR> a, int epi, int nr, int nt, hipStream_
     case 2:
         launch_epi<LF, LL, SR, 2>(a, epi, nt, cap, st);
         break;
+    case 3:
+        launch_epi<LF, LL, SR, 3>(a, epi, nt, cap, st);
+        break;
     case 4:
         launch_epi<LF, LL, SR, 4>(a, epi, nt, cap, st);
         break;

@@ -80,8 +80,8 @@ static KuQCDStatus apply(int nrhs, const void *src, void *dst, int parity, int m
         set_error("nrhs must be positive");
         return KUQCD_ERR_INVALID_PARAM;
     }
-    if (nr != 1 && nr != 2 && nr != 4 && nr != 8) {
-        set_error("nr must be 1, 2, 4 or 8");
+    if (nr != 1 && nr != 2 && nr != 3 && nr != 4 && nr != 8) {
+        set_error("nr must be 1, 2, 3, 4 or 8");
         return KUQCD_ERR_INVALID_PARAM;
     }
     if (nt != 64 && nt != 128 && nt != 256) {
@@ -206,19 +206,19 @@ std::pair<int, int> tuned_launch(int slot, int mode, int nrhs)
         return it->second;
     // Each blocking NR is timed once, on a single group of NR right-hand sides, and a launch
     // for nrhs systems is priced as ceil(nrhs / NR) such groups (the groups are independent
-    // workgroups of the same kernel). Blockings go up to the smallest power of two >= nrhs:
+    // workgroups of the same kernel). Blockings of 1, 2, 3, 4 and 8 are tried below 2 * nrhs:
     // the kernel pads a partial block with the last live right-hand side and writes nothing
-    // for it. FP64 stops at NR = 2 (register pressure). Each (NR, block size) is timed
+    // for it (three live systems in a block of four cost the padded fourth's arithmetic). FP64 stops at NR = 2 (register pressure). Each (NR, block size) is timed
     // without an occupancy cap and with caps of 5..2 waves per SIMD; the best cap is kept
     // (tuned_cap).
-    static const int kNr[4] = {1, 2, 4, 8}, kNt[3] = {64, 128, 256}, kCap[5] = {0, 5, 4, 3, 2};
+    static const int kNr[5] = {1, 2, 3, 4, 8}, kNt[3] = {64, 128, 256}, kCap[5] = {0, 5, 4, 3, 2};
     auto &tg = c.tune_group[{mode, g.Vh}];
     if (tg[0] == 0.0)
         tg.fill(-1.0);
     double best = 1e30;
     std::pair<int, int> pick{1, 256};
     bool l12 = false;
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 5; ++i) {
         const int nr = kNr[i];
         if (nr > 1 && (nr >= 2 * nrhs || (mode == KQ_MODE_FP64 && nr > 2)))
             continue;

@@ -2,6 +2,8 @@
 // FP64 / mixed-precision multishift CG with per-shift FP64 verification and
 // batched refinement.
 #include "kq_solve.h"
+#include "kq_blas.h"
+#include "kq_cgm_host.h"
 #include "kq_learned.h"
 
 #include <algorithm>
@@ -43,8 +45,9 @@ static void print_iters(int iter, int max_iter)
 static void set_caps(RunnerTuning &t, int slot, int mode, bool reliable)
 {
     const bool dbl = mode == KQ_MODE_FP64 || (mode == KQ_MODE_REFERENCE && ctx().fat[slot].prec == KUQCD_DOUBLE);
-    for (int i = 0; i < 4; ++i)
-        t.cap[i] = (dbl && i > 1) ? 0 : tuned_cap(slot, mode, 1 << i, t.nt);
+    static const int kNrs[5] = {1, 2, 3, 4, 8};
+    for (int i = 0; i < 5; ++i)
+        t.cap[i] = (dbl && kNrs[i] > 2) ? 0 : tuned_cap(slot, mode, kNrs[i], t.nt);
     if (reliable) {
         t.nt64 = tuned_launch(slot, KQ_MODE_FP64, 1).second;
         // KQ_RELIABLE_NT=64|128|256 fixes the FP64 term's block size (to test the runner
@@ -96,9 +99,104 @@ static KuQCDStatus solve_ref_t(int slot, LK fat, LK lng, void **x, void *b, KuQC
     return KUQCD_SUCCESS;
 }
 
+// ---------------------------------------------------------------- full operator
+// KUQCD_HISQ_DIRAC. The reference's kuqcd_inverter allocates the solve's fields on the parity of
+// matpc_type (kuqcd.cpp:200, 214-252) while the non-PC operator addresses both parities
+// (dirac_hisq.cpp:130, 422-428), so it reads and writes out of bounds. Here b and every x_k are
+// fields of both parities: V sites in the host order of a full field, the even block and then
+// the odd one (kuqcd.cpp:129-169 with i in [0, V)). CGM::Invert (inverter_cgm.cpp:17-87) runs
+// on them with A_k = sigma_k - D D, D the hopping term over all sites (dirac_hisq.cpp:409-462
+// with offseti = offsetg = 0): D maps the odd block to the even one and the even to the odd, so
+// D D acts on each parity by itself. The recurrence runs on the host in T (float for
+// KUQCD_SINGLE, as the reference's scalars); hopping terms and vector updates run on the device,
+// dot products as FP64 partial sums rounded to T. matpc_type is checked as the reference checks
+// it and not used otherwise.
+template <typename T, class LK> struct FullOps {
+    using V2 = c2_t<T>;
+    Geom g;
+    LK fat, lng;
+    hipStream_t st;
+    size_t n3, n;  // complex entries of one parity's field, of a field of both parities
+    V2 *r, *ap, *tmp, *p, *x;
+    double *part, *out;
+    double host_out = 0.0;
+
+    V2 *pk(int k) const { return p + (size_t)k * n; }
+    V2 *xk(int k) const { return x + (size_t)k * n; }
+    // dst = D src over both parities: dst(q) = D src(1 - q)
+    void dslash(const V2 *src, V2 *dst) const
+    {
+        for (int q = 0; q < 2; ++q) {
+            DslashArgs<LK, LK, T> a{};
+            a.g = g;
+            a.dst_parity = q;
+            a.fat = fat;
+            a.lng = lng;
+            a.src = Vec<T>{const_cast<V2 *>(src) + (size_t)(1 - q) * n3, g.Vh, n3};
+            a.dst = Vec<T>{dst + (size_t)q * n3, g.Vh, n3};
+            a.nrhs = 1;
+            launch_dslash(a, EPI_STORE, 1, kReproNT, st, 0);
+        }
+    }
+    T dot(const V2 *u, const V2 *v)
+    {
+        flat_dot<T>(u, v, n, part, out, st);
+        KQ_HIP_CHECK(hipMemcpyAsync(&host_out, out, sizeof(double), hipMemcpyDeviceToHost, st));
+        KQ_HIP_CHECK(hipStreamSynchronize(st));
+        return (T)host_out;
+    }
+    T norm2_r() { return dot(r, r); }
+    T apply(T sigma0)
+    {
+        dslash(pk(0), tmp);
+        dslash(tmp, ap);
+        flat_axpby<T>(pk(0), ap, n, sigma0, T(-1), st);  // Axmy: ap = sigma_0 p0 - D D p0
+        return dot(pk(0), ap);
+    }
+    void update_r(T b0) { flat_axpby<T>(ap, r, n, b0, T(1), st); }
+    void update_x(int k, T bk) { flat_axpby<T>(pk(k), xk(k), n, -bk, T(1), st); }
+    void update_p(int k, T zk, T ak) { flat_axpby<T>(r, pk(k), n, zk, ak, st); }
+};
+
+template <typename T, class LK>
+static KuQCDStatus solve_full_t(int slot, LK fat, LK lng, void **x, void *b, KuQCDInvertParam &p)
+{
+    using V2 = c2_t<T>;
+    Context &c = ctx();
+    const Geom g = slot_geom(slot);
+    const int ns = p.num_shift;
+    const size_t n3 = (size_t)3 * g.Vh, n = 2 * n3;
+    DevBuf bufs(sizeof(V2) * n * (3 + 2 * (size_t)ns)), part(sizeof(double) * (flat_dot_nblk(n) + 1)), stage;
+    V2 *base = bufs.as<V2>();
+    FullOps<T, LK> ops{g, fat, lng, c.stream, n3, n, base, base + n, base + 2 * n, base + 3 * n,
+                       base + (3 + (size_t)ns) * n, part.as<double>(), part.as<double>() + flat_dot_nblk(n)};
+    const Vec<T> rv{ops.r, g.Vh, n3};
+    for (int q = 0; q < 2; ++q)
+        upload_vec<T, T>(static_cast<const T *>(b) + (size_t)6 * g.Vh * q, rv, q, stage, c.stream);
+    for (int k = 0; k < ns; ++k)
+        KQ_HIP_CHECK(hipMemcpyAsync(ops.pk(k), ops.r, sizeof(V2) * n, hipMemcpyDeviceToDevice, c.stream));
+    KQ_HIP_CHECK(hipMemsetAsync(ops.x, 0, sizeof(V2) * n * ns, c.stream));
+    std::vector<T> sh(ns);
+    for (int k = 0; k < ns; ++k)
+        sh[k] = (T)p.shifts[k];  // kuqcd.cpp:270-272
+    const int iter = cgm_recurrence<T>(ops, ns, sh.data(), p.tol, p.max_iter);
+    for (int k = 0; k < ns; ++k)
+        for (int q = 0; q < 2; ++q)
+            download_vec<T, T>(Vec<T>{ops.xk(k), g.Vh, n3}, q, static_cast<T *>(x[k]) + (size_t)6 * g.Vh * q, stage,
+                               c.stream);
+    p.iter = iter;
+    print_iters(iter, p.max_iter);
+    return KUQCD_SUCCESS;
+}
+
 KuQCDStatus solve_reference(int slot, void **x, void *b, KuQCDInvertParam &p)
 {
     Context &c = ctx();
+    if (p.dslash_type == KUQCD_HISQ_DIRAC) {
+        if (c.fat[slot].prec == KUQCD_DOUBLE)
+            return solve_full_t<double, LinkD>(slot, link_d(c.fat[slot]), link_d(c.lng[slot]), x, b, p);
+        return solve_full_t<float, LinkF>(slot, link_f(c.fat[slot]), link_f(c.lng[slot]), x, b, p);
+    }
     if (c.fat[slot].prec == KUQCD_DOUBLE)
         return solve_ref_t<double, LinkD>(slot, link_d(c.fat[slot]), link_d(c.lng[slot]), x, b, p);
     return solve_ref_t<float, LinkF>(slot, link_f(c.fat[slot]), link_f(c.lng[slot]), x, b, p);

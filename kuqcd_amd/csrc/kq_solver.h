@@ -225,12 +225,12 @@ struct RunnerTuning {
     int nr = 1;      // right-hand sides per thread in the batched hopping term
     int nt = 256;    // threads per block of the hopping term
     int chunk = 8;   // iterations replayed between host checks
-    int cap[4] = {0, 0, 0, 0};  // occupancy cap of the hopping term for nr = 1, 2, 4, 8
+    int cap[5] = {0, 0, 0, 0, 0};  // occupancy cap of the hopping term for nr = 1, 2, 3, 4, 8
     int nt64 = 256;             // FP64 hopping term of the reliable updates: block size
     int cap64[2] = {0, 0};      // and occupancy cap for nr = 1, 2
     // nr by number of live systems (index clamped to the last entry); empty: always nr
     std::vector<int> nr_by_live;
-    int cap_for(int r) const { return cap[r >= 8 ? 3 : r >= 4 ? 2 : r >= 2 ? 1 : 0]; }
+    int cap_for(int r) const { return cap[r >= 8 ? 4 : r >= 4 ? 3 : r >= 3 ? 2 : r >= 2 ? 1 : 0]; }
     int pick_nr(int live) const
     {
         if (nr_by_live.empty())

@@ -97,7 +97,9 @@ def mdaggm(fat, lng, src, lat, parity):
 
 
 def cgm(fat, lng, lat, parity, shifts, b, tol, max_iter):
-    """CGM::Invert (inverter_cgm.cpp:17-87) in the dtype of b; returns (x list, iterations)."""
+    """CGM::Invert (inverter_cgm.cpp:17-87) in the dtype of b; returns (x list, iterations).
+    parity 0 or 1: the even-odd operator on fields of that parity; parity -1: the full operator
+    (DiracHISQ with KUQCD_HISQ_DIRAC) on fields of both parities, even block first."""
     dt = b.dtype
     sh = np.ascontiguousarray(shifts, dtype=dt)
     xs = [np.empty_like(b) for _ in range(len(sh))]

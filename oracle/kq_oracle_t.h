@@ -159,8 +159,24 @@ REAL FN(kqo_norm2)(const REAL *x, long n)
     return FN(kqo_realdot)(x, x, n);
 }
 
+/* D D over fields of both parities (even block, then odd): the full operator of a
+ * KUQCD_HISQ_DIRAC DiracHISQ (dirac_hisq.cpp:409-462 with offseti = offsetg = 0) maps the odd
+ * block to the even one and back, so its square is MdaggM of each block on its own parity. */
+static void FN(mdaggm_any)(const REAL *fat, const REAL *lng, const REAL *src, REAL *dst, REAL *tmp, const int L[4],
+                           int parity)
+{
+    const long n6 = 6L * L[0] * L[1] * L[2] * L[3] / 2;
+    if (parity >= 0) {
+        FN(kqo_mdaggm)(fat, lng, src, dst, tmp, L, parity);
+        return;
+    }
+    FN(kqo_mdaggm)(fat, lng, src, dst, tmp, L, 0);
+    FN(kqo_mdaggm)(fat, lng, src + n6, dst + n6, tmp + n6, L, 1);
+}
+
 /* CGM::Invert, inverter_cgm.cpp:17-87: Jegerlehner multishift CG on
  *   A_0 = shifts[0] - D D,  A_k = shifts[0] + shifts[k] - D D (k >= 1),
+ * over fields of one parity (parity 0 or 1) or of both (parity -1, mdaggm_any),
  * scalars in REAL, stop when zeta_k^2 |r|^2 < tol for the last active shift
  * (absolute squared residual), converged shifts dropped from the end.
  * The literal 1.0 in the zeta recurrence is a double, as in the reference
@@ -170,7 +186,7 @@ int FN(kqo_cgm)(const REAL *fat, const REAL *lng, const int L[4], int parity, in
                 const REAL *b, REAL **x, double tol, int max_iter)
 {
     const long Vh = (long)L[0] * L[1] * L[2] * L[3] / 2;
-    const long n = 6 * Vh;
+    const long n = (parity < 0 ? 12 : 6) * Vh;
     const size_t bytes = sizeof(REAL) * (size_t)n;
     int maxTerm = nshift;
     REAL *a = (REAL *)calloc((size_t)nshift, sizeof(REAL));
@@ -196,7 +212,7 @@ int FN(kqo_cgm)(const REAL *fat, const REAL *lng, const int L[4], int parity, in
     do {
         iter++;
         memcpy(pi0, p[0], bytes);
-        FN(kqo_mdaggm)(fat, lng, pi0, s, tmp, L, parity);
+        FN(mdaggm_any)(fat, lng, pi0, s, tmp, L, parity);
         FN(kqo_axmy)(s, pi0, shifts[0], n);
         pAp = FN(kqo_realdot)(pi0, s, n);
         B[0] = -rNorm2 / pAp;

@@ -12,6 +12,12 @@
     give the same operator.
   * the float reference method (KUQCD_SINGLE, Q4: float recurrence scalars) over
     hundreds of iterations against the oracle's float CGM (inverter_cgm.cpp:17-87).
+  * KUQCD_HISQ_DIRAC (the non-PC operator), which the reference runs on fields of one parity
+    that its operator indexes as fields of both (out of bounds, dirac_hisq.cpp:130, 422): here
+    b and x_k hold both parities (V sites, even block first) and CGM::Invert runs with the full
+    operator; against the oracle's CGM on fields of both parities (pinned by dense solves in
+    tests/test_oracle.py) in double and float, and each parity of the solution against the
+    library's own even-odd solve of that parity. The extensions reject it.
 
 Tolerances: a pair in BATCH and NORMAL order, and halo depth 0 and 3, run the
 same kernels with the same fixed reduction order, so their solutions are
@@ -177,3 +183,42 @@ def test_float_reference_method_matches_oracle_float_cgm(gpu, rel):
         print(f"shift {k}: vs oracle float {e_o:.2e}, vs oracle double {e_d:.2e}, oracle float vs double "
               f"{rel_err(xo[k], xd[k]):.2e}", flush=True)
         assert e_o <= 5e-6 and e_d <= 5e-6, k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [8, 4])
+def test_full_operator_solve(gpu, prec):
+    """The 8^4 problem of the CPU-runnable workload with a source on both parities."""
+    kq = gpu
+    lat = (8, 8, 8, 8)
+    fat, lng, _, shifts = _problem(kq, lat, 71, prec)
+    dt = fat.dtype
+    b = np.concatenate([kq.gen_source(lat, 0, seed=73), kq.gen_source(lat, 1, seed=74)]).astype(dt)
+    Vh = b.shape[0] // 2
+    _load(kq, fat, lng, lat, prec, kq.KUQCD_NORMAL_ORDER)
+    rel = 1e-5 if prec == kq.KUQCD_SINGLE else 1e-10
+    tol = rel * rel * float(np.sum(b.astype(np.float64) ** 2))
+    x = [np.zeros_like(b) for _ in shifts]
+    prm = kq.InvertParam(shifts=shifts, tol=tol, max_iter=5000, prec=prec, dslash_type=kq.KUQCD_HISQ_DIRAC)
+    print("full operator, prec", prec, flush=True)
+    assert kq.kuqcd_solve_multishift(x, b, prm) == kq.KUQCD_SUCCESS, kq.last_error()
+    xo, ito = oracle.cgm(fat, lng, lat, -1, shifts.astype(dt), b, tol, 5000)
+    print(f"iterations gpu {prm.iter}, oracle {ito}", flush=True)
+    assert prm.iter > 50 and abs(prm.iter - ito) <= max(2, ito // 100), (prm.iter, ito)
+    for k in range(len(shifts)):
+        assert rel_err(x[k], xo[k]) <= (5e-6 if prec == kq.KUQCD_SINGLE else 1e-8), k
+    # each parity of the solution solves that parity's even-odd system
+    for q, mt in ((0, kq.KUQCD_EVEN_EVEN), (1, kq.KUQCD_ODD_ODD)):
+        bq = np.ascontiguousarray(b[q * Vh:(q + 1) * Vh])
+        rq = rel if prec == kq.KUQCD_SINGLE else rel / 10
+        tq = rq * rq * float(np.sum(bq.astype(np.float64) ** 2))
+        xq = [np.zeros_like(bq) for _ in shifts]
+        pq = kq.InvertParam(shifts=shifts, tol=tq, max_iter=5000, prec=prec, matpc_type=mt)
+        assert kq.kuqcd_solve_multishift(xq, bq, pq) == kq.KUQCD_SUCCESS, kq.last_error()
+        for k in range(len(shifts)):
+            assert rel_err(x[k][q * Vh:(q + 1) * Vh], xq[k]) <= (2e-5 if prec == kq.KUQCD_SINGLE else 1e-8), (q, k)
+    # the extended entry points solve the even-odd operator only
+    if prec == kq.KUQCD_DOUBLE:
+        st, _ = kq.solve_ex(x, b, prm, kq.default_options())
+        assert st == kq.KUQCD_ERR_UNSUPPORTED
+    kq.kuqcd_gauge_free()

@@ -72,7 +72,7 @@ def test_dslash_batched_equals_single(gpu, mode):
     single = np.empty_like(src)
     for r in range(nrhs):
         assert gpu.dslash(np.ascontiguousarray(src[r]), single[r], 0, mode) == 0
-    nrs = [1, 2] if mode == 1 else [1, 2, 4, 8]
+    nrs = [1, 2] if mode == 1 else [1, 2, 3, 4, 8]
     for nr in nrs:
         for nt in (64, 128, 256):
             out = np.empty_like(src)

@@ -10,7 +10,8 @@ Each check is chosen so that a plausible restatement mistake fails it:
   * gauge covariance under U(x) -> g(x) U(x) g(x+mu)^+, N(x) -> g(x) N(x) g(x+3mu)^+
   * an independent dense-matrix construction from full-lattice coordinates
   * CGM against dense solves of every shifted system, with the reference's
-    shift convention (inverter_cgm.cpp:52: system k is shifts[0] + shifts[k] - DD)
+    shift convention (inverter_cgm.cpp:52: system k is shifts[0] + shifts[k] - DD), on one
+    parity and on fields of both parities with the full operator
 """
 import numpy as np
 import pytest
@@ -238,6 +239,33 @@ def test_cgm_matches_dense_solves(dtype, tolr):
     bc = _cplx(b).reshape(-1)
     for k, s in enumerate(sig_abs):
         xk = np.linalg.solve(K + s * np.eye(K.shape[0]), bc)
+        assert rel_err(x[k], _real(xk).reshape(x[k].shape)) < tolr, k
+
+
+@pytest.mark.parametrize("dtype,tolr", [(np.float64, 1e-9), (np.float32, 2e-4)])
+def test_cgm_full_operator_matches_dense_solves(dtype, tolr):
+    """parity -1: fields of both parities (even block first) and the full operator of a
+    KUQCD_HISQ_DIRAC DiracHISQ, against dense solves of sigma_k - D D built from full-lattice
+    coordinates (D D is block diagonal: -D_eo D_oe on the even block, -D_oe D_eo on the odd)."""
+    lat = (4, 4, 4, 4)
+    rng = np.random.default_rng(9)
+    fat = _random_links(rng, lat, 0.9)
+    lng = _random_links(rng, lat, 1 / 24)
+    V = int(np.prod(lat))
+    D0, D1 = dense_D(fat, lng, lat, 0), dense_D(fat, lng, lat, 1)
+    n = D0.shape[0]
+    K = np.zeros((2 * n, 2 * n), dtype=complex)
+    K[:n, :n] = -D0 @ D1
+    K[n:, n:] = -D1 @ D0
+    b = rng.standard_normal((V, 3, 2))
+    sig_abs = np.array([0.0025, 0.0125, 0.1025, 1.0025])
+    shifts = np.concatenate([[sig_abs[0]], sig_abs[1:] - sig_abs[0]])
+    tol = (1e-12 if dtype == np.float64 else 1e-6) ** 2 * float(np.sum(b * b))
+    x, it = oracle.cgm(fat.astype(dtype), lng.astype(dtype), lat, -1, shifts.astype(dtype), b.astype(dtype), tol, 5000)
+    assert 1 < it < 5000 and x[0].shape == b.shape
+    bc = _cplx(b).reshape(-1)
+    for k, s in enumerate(sig_abs):
+        xk = np.linalg.solve(K + s * np.eye(2 * n), bc)
         assert rel_err(x[k], _real(xk).reshape(x[k].shape)) < tolr, k
 
 
