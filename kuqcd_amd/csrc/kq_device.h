@@ -101,29 +101,19 @@ inline uint32_t tile_slab_limit()
     return v;
 }
 
-__host__ inline Geom make_geom(const int L[4])
+// Tile split of the hopping terms' site order (tile_site): 8 tiles of (y ranges, z ranges,
+// t ranges), the first split that divides the lattice with an (x, y) slab of at most slab_limit
+// sites of one parity, else the first that divides it
+__host__ inline void set_tiles(Geom &g, uint64_t slab_limit)
 {
-    Geom g;
-    for (int i = 0; i < 4; ++i)
-        g.L[i] = L[i];
-    g.lxh = L[0] / 2;
-    g.Vh = (uint32_t)((uint64_t)L[0] * L[1] * L[2] * L[3] / 2);
-    g.stride[0] = 1;
-    g.stride[1] = (uint32_t)g.lxh;
-    g.stride[2] = (uint32_t)g.lxh * L[1];
-    g.stride[3] = (uint32_t)g.lxh * L[1] * L[2];
-    g.dx = FastDiv((uint32_t)g.lxh);
-    g.dy = FastDiv((uint32_t)L[1]);
-    g.dz = FastDiv((uint32_t)L[2]);
-    // XCD splits (y ranges, z ranges, t ranges), 8 blocks each: the first that divides the
-    // lattice with a slab of at most kTileSlab sites, else the first that divides it
+    const int *L = g.L;
     static const int kSplit[3][3] = {{1, 4, 2}, {2, 4, 1}, {4, 2, 1}};
     int pick = -1;
     for (int i = 0; i < 3; ++i) {
         const int yp = kSplit[i][0], zp = kSplit[i][1], tp = kSplit[i][2];
         if (L[1] % yp != 0 || L[2] % zp != 0 || L[3] % tp != 0)
             continue;
-        const bool fits = (uint64_t)g.lxh * (L[1] / yp) <= tile_slab_limit();
+        const bool fits = (uint64_t)g.lxh * (L[1] / yp) <= slab_limit;
         if (pick < 0 || fits) {
             pick = i;
             if (fits)
@@ -142,6 +132,23 @@ __host__ inline Geom make_geom(const int L[4])
     g.slab = (uint32_t)g.lxh * g.yc;
     g.dslab = FastDiv(g.slab > 0 ? g.slab : 1);
     g.dzc = FastDiv(g.zc > 0 ? g.zc : 1);
+}
+
+__host__ inline Geom make_geom(const int L[4])
+{
+    Geom g;
+    for (int i = 0; i < 4; ++i)
+        g.L[i] = L[i];
+    g.lxh = L[0] / 2;
+    g.Vh = (uint32_t)((uint64_t)L[0] * L[1] * L[2] * L[3] / 2);
+    g.stride[0] = 1;
+    g.stride[1] = (uint32_t)g.lxh;
+    g.stride[2] = (uint32_t)g.lxh * L[1];
+    g.stride[3] = (uint32_t)g.lxh * L[1] * L[2];
+    g.dx = FastDiv((uint32_t)g.lxh);
+    g.dy = FastDiv((uint32_t)L[1]);
+    g.dz = FastDiv((uint32_t)L[2]);
+    set_tiles(g, tile_slab_limit());
     return g;
 }
 
