@@ -24,6 +24,7 @@ LIB = LIBDIR / "libkuqcd.so"
 
 SOURCES = [
     "kq_dslash_fp64.hip",
+    "kq_dslash_fp64r.hip",
     "kq_dslash_fp32.hip",
     "kq_dslash_mixed.hip",
     "kq_dslash_16.hip",

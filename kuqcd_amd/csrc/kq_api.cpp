@@ -85,6 +85,12 @@ LinkD link_d(const GaugeSlot &s)
     const uint32_t Vh = (uint32_t)((uint64_t)s.lat[0] * s.lat[1] * s.lat[2] * s.lat[3] / 2);
     return LinkD{s.kf.as<char>(), nb64_of(Vh)};
 }
+LinkD14 link_d14(const GaugeSlot &s)
+{
+    const uint32_t Vh = (uint32_t)((uint64_t)s.lat[0] * s.lat[1] * s.lat[2] * s.lat[3] / 2);
+    return LinkD14{s.kf14.as<char>(), nb64_of(Vh)};
+}
+
 LinkF link_f(const GaugeSlot &s)
 {
     const uint32_t Vh = (uint32_t)((uint64_t)s.lat[0] * s.lat[1] * s.lat[2] * s.lat[3] / 2);
@@ -204,7 +210,7 @@ static KuQCDStatus init_from_stdin()
 
 // ---------------------------------------------------------------- load
 template <typename HR>
-static void upload_links(const HR *h, GaugeSlot &slot, double scale, int prec)
+static void upload_links(const HR *h, GaugeSlot &slot, double scale, int prec, bool long_links)
 {
     Context &c = ctx();
     const uint64_t V = (uint64_t)slot.lat[0] * slot.lat[1] * slot.lat[2] * slot.lat[3];
@@ -228,6 +234,19 @@ static void upload_links(const HR *h, GaugeSlot &slot, double scale, int prec)
     if (prec == KUQCD_DOUBLE) {
         slot.kf.alloc(link_field_bytes<LinkD>(Vh));
         canon_to_linkD(slot.soa.as<double2>(), slot.kf.as<char>(), Vh, c.stream);
+        // long links also as 14 reals when every one is a complex multiple of an SU(3) matrix
+        if (long_links) {
+            DevBuf fail(sizeof(int));
+            KQ_HIP_CHECK(hipMemsetAsync(fail.as<void>(), 0, sizeof(int), c.stream));
+            slot.kf14.alloc(link_field_bytes<LinkD14>(Vh));
+            canon_to_linkD14(slot.soa.as<double2>(), slot.kf14.as<char>(), Vh, fail.as<int>(), c.stream);
+            int nfail = 0;
+            KQ_HIP_CHECK(hipMemcpyAsync(&nfail, fail.as<void>(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+            KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+            slot.d14 = (nfail == 0);
+            if (!slot.d14)
+                slot.kf14.release();
+        }
     } else {
         slot.kf.alloc(link_field_bytes<LinkF>(Vh));
         canon_to_linkF<float>(slot.soa.as<float2>(), slot.kf.as<char>(), Vh, c.stream);
@@ -306,15 +325,18 @@ static KuQCDStatus load_gauge(void *h_gauge, KuQCDGaugeParam *param)
     gs.loaded = false;
     gs.soa.release();
     gs.kf.release();
+    gs.kf14.release();
+    gs.d14 = false;
     free_derived(slot);
     for (int i = 0; i < 4; ++i)
         gs.lat[i] = param->lat[i];
     gs.prec = param->prec;
     const double scale = (param->link_type == KUQCD_FAT_LINK) ? 0.5 : (-1.0 / 48.0);
     if (param->prec == KUQCD_DOUBLE)
-        upload_links<double>(static_cast<const double *>(h_gauge), gs, scale, KUQCD_DOUBLE);
+        upload_links<double>(static_cast<const double *>(h_gauge), gs, scale, KUQCD_DOUBLE,
+                             param->link_type == KUQCD_LONG_LINK);
     else
-        upload_links<float>(static_cast<const float *>(h_gauge), gs, scale, KUQCD_SINGLE);
+        upload_links<float>(static_cast<const float *>(h_gauge), gs, scale, KUQCD_SINGLE, false);
     gs.loaded = true;
     return KUQCD_SUCCESS;
 }

@@ -24,6 +24,8 @@ struct GaugeSlot {
     unsigned lat[4] = {0, 0, 0, 0};
     DevBuf soa;  // canonical, double2 or float2
     DevBuf kf;   // LinkD (double) or LinkF (float) format
+    bool d14 = false;  // double long links also held as LinkD14 in kf14
+    DevBuf kf14;
 };
 
 // Lower-precision kernel formats of a double gauge pair, made on first use.
@@ -69,6 +71,11 @@ int active_slot();
 Geom slot_geom(int slot);
 
 LinkD link_d(const GaugeSlot &s);
+LinkD14 link_d14(const GaugeSlot &s);
+// fn(LinkD fat, lng) with the long links of the slot's double pair as LinkD14 when every one
+// has that form (GaugeSlot::d14), else as LinkD: the FP64 hopping term of the solvers, the FP64
+// verification and the operator entry points
+template <class Fn> inline void with_fp64_links(int slot, Fn &&fn);
 LinkF link_f(const GaugeSlot &s);
 // derived formats of a double slot (created on demand)
 DerivedFormats &derived(int slot, bool want_f32, bool want_f16);
@@ -110,6 +117,15 @@ template <class Fn> bool with_mixed_links(int slot, int mode, Fn &&fn)
     else
         fn(as_linkF(d.fat_f, Vh), as_linkF(d.lng_f, Vh));
     return d.f32_long12;
+}
+
+template <class Fn> inline void with_fp64_links(int slot, Fn &&fn)
+{
+    Context &c = ctx();
+    if (c.lng[slot].d14)
+        fn(link_d(c.fat[slot]), link_d14(c.lng[slot]));
+    else
+        fn(link_d(c.fat[slot]), link_d(c.lng[slot]));
 }
 
 // device address of the no-allocation entry point's static scratch array on the current

@@ -91,6 +91,41 @@ struct LinkD {  // FP64, 18 reals: 9 chunks of 16 B (one complex each)
     }
 };
 
+// FP64 long links as 14 reals: rows 0 and 1 of an SU(3) matrix V and a complex scale w, U = w V
+// with row 2 of V = conj(row0 x row1). A HISQ long link is c_Naik times a product of three U(3)
+// links, so it has this form with w^3 = det U (canon_to_linkD14 checks every link and the
+// library keeps LinkD otherwise). 7 chunks of 16 B: complex 0..5 of V, then w.
+struct LinkD14 {
+    using R = double;
+    static constexpr int kBytes = 112;
+    static constexpr size_t kBlock = 64 * kBytes;
+    const char *p;
+    uint32_t nb64;
+    __device__ inline void load(int q, int mu, uint32_t s, double2 u[9]) const
+    {
+        const kq_v2d *b = reinterpret_cast<const kq_v2d *>(p + link_block_offset(q, mu, s, nb64, kBlock)) + (s & 63);
+        double2 v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const kq_v2d c = ld_nt(b + 64 * k);
+            v[k] = mk<double2>(c.x, c.y);
+        }
+        const kq_v2d w = ld_nt(b + 64 * 6);
+        // row 2 of V = conj(row0 x row1), then U = w V
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            const double2 a1 = v[c1], b2 = v[3 + c2], a2 = v[c2], b1 = v[3 + c1];
+            const double re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
+            const double im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
+            u[6 + c] = mk<double2>(w.x * re + w.y * im, w.y * re - w.x * im);  // w * conj(re + i im)
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            u[k] = mk<double2>(w.x * v[k].x - w.y * v[k].y, w.x * v[k].y + w.y * v[k].x);
+    }
+};
+
 struct LinkF {  // FP32, 18 reals: 4 chunks of 16 B (complex 0..7) + 1 of 8 B (complex 8)
     using R = float;
     static constexpr int kBytes = 72;

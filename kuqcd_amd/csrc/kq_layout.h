@@ -21,6 +21,9 @@ void gauge_host_to_soa(const HR *raw, size_t i0, size_t n, c2_t<LR> *dst, uint32
 // canon[((q*4 + mu)*9 + k)*Vh + s] (complex):
 // FP64 -> LinkD; FP64 or FP32 -> LinkF (rounding to nearest)
 void canon_to_linkD(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st);
+// FP64 long links as LinkD14; *fail (device int, not cleared here) counts the links that are not
+// a complex multiple of an SU(3) matrix to 1e-13
+void canon_to_linkD14(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);
 template <typename CR> void canon_to_linkF(const c2_t<CR> *canon, char *dst, uint32_t Vh, hipStream_t st);
 // FP64 -> LinkH: 16-bit fixed-point entries with one FP32 scale per link (max |entry| / 32767)
 void canon_to_linkH(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st);

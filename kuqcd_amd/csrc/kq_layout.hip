@@ -194,6 +194,61 @@ __global__ void k_canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh,
     reinterpret_cast<float *>(b + 1536)[l] = (float)bsc;
 }
 
+// LinkD14: w = principal cube root of det U (so V = U / w has det 1), rows 0 and 1 of V and w.
+// A link counts in *fail when the reconstruction w (V0, V1, conj(V0 x V1)), formed as
+// LinkD14::load forms it, differs from U by more than 1e-13 max|U_ij| in any entry: U is then not
+// a complex multiple of an SU(3) matrix to FP64 accuracy (or is zero).
+__global__ void k_canon_to_linkD14(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64, int *fail)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)8 * Vh)
+        return;
+    const uint32_t qm = (uint32_t)(t / Vh), s = (uint32_t)(t - (size_t)qm * Vh);
+    double2 m[9];
+    double amax = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        m[k] = canon[((size_t)qm * 9 + k) * Vh + s];
+        amax = fmax(amax, fmax(fabs(m[k].x), fabs(m[k].y)));
+    }
+    const double2 d = det3(m);
+    const double r = cbrt(hypot(d.x, d.y)), ph = atan2(d.y, d.x) / 3.0;
+    const double2 w = mk<double2>(r * cos(ph), r * sin(ph));
+    const double inv = r > 0.0 ? 1.0 / (r * r) : 0.0;
+    double2 v[6];
+    for (int k = 0; k < 6; ++k)  // m / w = m conj(w) / |w|^2
+        v[k] = mk<double2>((m[k].x * w.x + m[k].y * w.y) * inv, (m[k].y * w.x - m[k].x * w.y) * inv);
+    double err = 0.0;
+    for (int k = 0; k < 6; ++k) {
+        const double2 u = mk<double2>(w.x * v[k].x - w.y * v[k].y, w.x * v[k].y + w.y * v[k].x);
+        err = fmax(err, fmax(fabs(u.x - m[k].x), fabs(u.y - m[k].y)));
+    }
+    for (int c = 0; c < 3; ++c) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        const double2 a1 = v[c1], b2 = v[3 + c2], a2 = v[c2], b1 = v[3 + c1];
+        const double re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
+        const double im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
+        const double2 u = mk<double2>(w.x * re + w.y * im, w.y * re - w.x * im);
+        err = fmax(err, fmax(fabs(u.x - m[6 + c].x), fabs(u.y - m[6 + c].y)));
+    }
+    if (!(r > 0.0 && err <= 1e-13 * amax))
+        atomicAdd(fail, 1);
+    double2 *b = reinterpret_cast<double2 *>(dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkD14::kBlock)) +
+                 (s & 63);
+    for (int k = 0; k < 6; ++k)
+        b[64 * k] = v[k];
+    b[64 * 6] = w;
+}
+
+void canon_to_linkD14(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+{
+    const size_t n = (size_t)8 * Vh;
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_canon_to_linkD14, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
+                       nb64_of(Vh), fail);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
 void canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
 {
     const size_t n = (size_t)8 * Vh;

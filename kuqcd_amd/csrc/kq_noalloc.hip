@@ -66,8 +66,8 @@ static size_t noalloc_mdaggm_bytes(uint32_t Vh, size_t real_bytes)
     return 2 * vec + round_up(sizeof(double) * dslash_nblk(Vh, kReproNT)) + 256;
 }
 
-template <class LK, typename R>
-static void mdaggm_noalloc(LK fat, LK lng, const Geom &g, const R *src, R *dst, int parity, double sigma, hipStream_t st)
+template <class LF, class LL, typename R>
+static void mdaggm_noalloc(LF fat, LL lng, const Geom &g, const R *src, R *dst, int parity, double sigma, hipStream_t st)
 {
     using V2 = c2_t<R>;
     const size_t vec = round_up(sizeof(V2) * 3 * (size_t)g.Vh);
@@ -83,7 +83,7 @@ static void mdaggm_noalloc(LK fat, LK lng, const Geom &g, const R *src, R *dst, 
     vec_host_to_soa<R, R>(src, sv, 0, st);
     hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, st, sig, sigma);
     KQ_HIP_CHECK(hipGetLastError());
-    DslashArgs<LK, LK, R> a{};
+    DslashArgs<LF, LL, R> a{};
     a.g = g;
     a.fat = fat;
     a.lng = lng;
@@ -154,12 +154,17 @@ extern "C" KuQCDStatus kq_mdaggm_noalloc(const void *src, void *dst, int parity,
             set_error("kq_mdaggm_noalloc: src and dst overlap");
             return KUQCD_ERR_INVALID_PARAM;
         }
+        // the same link formats as kq_mdaggm (with_fp64_links), so the two agree bitwise
         if (dbl)
-            mdaggm_noalloc<LinkD, double>(link_d(c.fat[slot]), link_d(c.lng[slot]), g, static_cast<const double *>(src),
-                                          static_cast<double *>(dst), parity, sigma, c.stream);
+            with_fp64_links(slot, [&](auto fat, auto lng) {
+                mdaggm_noalloc<decltype(fat), decltype(lng), double>(fat, lng, g, static_cast<const double *>(src),
+                                                                     static_cast<double *>(dst), parity, sigma,
+                                                                     c.stream);
+            });
         else
-            mdaggm_noalloc<LinkF, float>(link_f(c.fat[slot]), link_f(c.lng[slot]), g, static_cast<const float *>(src),
-                                         static_cast<float *>(dst), parity, sigma, c.stream);
+            mdaggm_noalloc<LinkF, LinkF, float>(link_f(c.fat[slot]), link_f(c.lng[slot]), g,
+                                                static_cast<const float *>(src), static_cast<float *>(dst), parity,
+                                                sigma, c.stream);
         return KUQCD_SUCCESS;
     } catch (const std::exception &e) {
         set_error(e.what());

@@ -93,8 +93,10 @@ static KuQCDStatus apply(int nrhs, const void *src, void *dst, int parity, int m
     const bool dbl = c.fat[slot].prec == KUQCD_DOUBLE;
     if (mode == KQ_MODE_REFERENCE || mode == KQ_MODE_FP64) {
         if (dbl)
-            apply_t<LinkD, LinkD, double, double>(link_d(c.fat[slot]), link_d(c.lng[slot]), g, nrhs, src, dst, parity,
-                                                  nr, nt, mdaggm, sigma);
+            with_fp64_links(slot, [&](auto fat, auto lng) {
+                apply_t<decltype(fat), decltype(lng), double, double>(fat, lng, g, nrhs, src, dst, parity, nr, nt,
+                                                                      mdaggm, sigma);
+            });
         else if (mode == KQ_MODE_REFERENCE)
             apply_t<LinkF, LinkF, float, float>(link_f(c.fat[slot]), link_f(c.lng[slot]), g, nrhs, src, dst, parity,
                                                 nr, nt, mdaggm, sigma);
@@ -156,6 +158,7 @@ static double time_dslash(LF fat, LL lng, const Geom &g, int nrhs, int nr, int n
 
 // Algorithmic bytes per output site: every link of the lattice once
 // (8 fat + 8 long per site), one source and one result vector per right-hand side.
+// long12: long links in 12 reals + scale (FP32 / 16-bit formats) or 14 reals (FP64, LinkD14).
 static double site_bytes(int mode, bool dbl_gauge, bool long12, int nrhs)
 {
     double bl_f, bl_l, bv;
@@ -168,7 +171,8 @@ static double site_bytes(int mode, bool dbl_gauge, bool long12, int nrhs)
         bl_l = (long12 && mode == KQ_MODE_MIXED_FP32) ? 48 + 4 : 72;
         bv = 24;
     } else {
-        bl_f = bl_l = 144;
+        bl_f = 144;
+        bl_l = long12 ? 112 : 144;
         bv = 48;
     }
     return 8 * bl_f + 8 * bl_l + nrhs * 2 * bv;
@@ -182,9 +186,14 @@ static double time_mode(int slot, int mode, int nrhs, int nr, int nt, int iters,
     const bool dbl = c.fat[slot].prec == KUQCD_DOUBLE;
     *long12 = false;
     if (mode == KQ_MODE_REFERENCE || mode == KQ_MODE_FP64) {
-        if (dbl)
-            return time_dslash<LinkD, LinkD, double>(link_d(c.fat[slot]), link_d(c.lng[slot]), g, nrhs, nr, nt, iters,
-                                                     cap);
+        if (dbl) {
+            double ms = 0.0;
+            *long12 = c.lng[slot].d14;
+            with_fp64_links(slot, [&](auto fat, auto lng) {
+                ms = time_dslash<decltype(fat), decltype(lng), double>(fat, lng, g, nrhs, nr, nt, iters, cap);
+            });
+            return ms;
+        }
         return time_dslash<LinkF, LinkF, float>(link_f(c.fat[slot]), link_f(c.lng[slot]), g, nrhs, nr, nt, iters, cap);
     }
     double ms = 0.0;

@@ -63,8 +63,8 @@ static void set_caps(RunnerTuning &t, int slot, int mode, bool reliable)
 }
 
 // ---------------------------------------------------------------- reference method
-template <typename T, class LK>
-static KuQCDStatus solve_ref_t(int slot, LK fat, LK lng, void **x, void *b, KuQCDInvertParam &p)
+template <typename T, class LF, class LL>
+static KuQCDStatus solve_ref_t(int slot, LF fat, LL lng, void **x, void *b, KuQCDInvertParam &p)
 {
     Context &c = ctx();
     const Geom g = slot_geom(slot);
@@ -75,7 +75,7 @@ static KuQCDStatus solve_ref_t(int slot, LK fat, LK lng, void **x, void *b, KuQC
     tune.nt = kReproNT;
     set_caps(tune, slot, KQ_MODE_REFERENCE, false);
     tune.chunk = 16;
-    CgmRunner<T, T, LK, LK> run(g, parity, fat, lng, 1, ns, c.stream, tune);
+    CgmRunner<T, T, LF, LL> run(g, parity, fat, lng, 1, ns, c.stream, tune);
     // FP64 solutions take their updates per segment of recorded residuals: the same iterations
     // and scalars as updating every shift each iteration, with a quarter of the vector traffic
     if constexpr (std::is_same<T, double>::value)
@@ -111,10 +111,11 @@ static KuQCDStatus solve_ref_t(int slot, LK fat, LK lng, void **x, void *b, KuQC
 // KUQCD_SINGLE, as the reference's scalars); hopping terms and vector updates run on the device,
 // dot products as FP64 partial sums rounded to T. matpc_type is checked as the reference checks
 // it and not used otherwise.
-template <typename T, class LK> struct FullOps {
+template <typename T, class LF, class LL> struct FullOps {
     using V2 = c2_t<T>;
     Geom g;
-    LK fat, lng;
+    LF fat;
+    LL lng;
     hipStream_t st;
     size_t n3, n;  // complex entries of one parity's field, of a field of both parities
     V2 *r, *ap, *tmp, *p, *x;
@@ -127,7 +128,7 @@ template <typename T, class LK> struct FullOps {
     void dslash(const V2 *src, V2 *dst) const
     {
         for (int q = 0; q < 2; ++q) {
-            DslashArgs<LK, LK, T> a{};
+            DslashArgs<LF, LL, T> a{};
             a.g = g;
             a.dst_parity = q;
             a.fat = fat;
@@ -158,8 +159,8 @@ template <typename T, class LK> struct FullOps {
     void update_p(int k, T zk, T ak) { flat_axpby<T>(r, pk(k), n, zk, ak, st); }
 };
 
-template <typename T, class LK>
-static KuQCDStatus solve_full_t(int slot, LK fat, LK lng, void **x, void *b, KuQCDInvertParam &p)
+template <typename T, class LF, class LL>
+static KuQCDStatus solve_full_t(int slot, LF fat, LL lng, void **x, void *b, KuQCDInvertParam &p)
 {
     using V2 = c2_t<T>;
     Context &c = ctx();
@@ -168,7 +169,7 @@ static KuQCDStatus solve_full_t(int slot, LK fat, LK lng, void **x, void *b, KuQ
     const size_t n3 = (size_t)3 * g.Vh, n = 2 * n3;
     DevBuf bufs(sizeof(V2) * n * (3 + 2 * (size_t)ns)), part(sizeof(double) * (flat_dot_nblk(n) + 1)), stage;
     V2 *base = bufs.as<V2>();
-    FullOps<T, LK> ops{g, fat, lng, c.stream, n3, n, base, base + n, base + 2 * n, base + 3 * n,
+    FullOps<T, LF, LL> ops{g, fat, lng, c.stream, n3, n, base, base + n, base + 2 * n, base + 3 * n,
                        base + (3 + (size_t)ns) * n, part.as<double>(), part.as<double>() + flat_dot_nblk(n)};
     const Vec<T> rv{ops.r, g.Vh, n3};
     for (int q = 0; q < 2; ++q)
@@ -192,14 +193,20 @@ static KuQCDStatus solve_full_t(int slot, LK fat, LK lng, void **x, void *b, KuQ
 KuQCDStatus solve_reference(int slot, void **x, void *b, KuQCDInvertParam &p)
 {
     Context &c = ctx();
-    if (p.dslash_type == KUQCD_HISQ_DIRAC) {
-        if (c.fat[slot].prec == KUQCD_DOUBLE)
-            return solve_full_t<double, LinkD>(slot, link_d(c.fat[slot]), link_d(c.lng[slot]), x, b, p);
-        return solve_full_t<float, LinkF>(slot, link_f(c.fat[slot]), link_f(c.lng[slot]), x, b, p);
+    const bool full = p.dslash_type == KUQCD_HISQ_DIRAC;
+    if (c.fat[slot].prec == KUQCD_DOUBLE) {
+        KuQCDStatus st = KUQCD_SUCCESS;
+        with_fp64_links(slot, [&](auto fat, auto lng) {
+            using LF = decltype(fat);
+            using LL = decltype(lng);
+            st = full ? solve_full_t<double, LF, LL>(slot, fat, lng, x, b, p)
+                      : solve_ref_t<double, LF, LL>(slot, fat, lng, x, b, p);
+        });
+        return st;
     }
-    if (c.fat[slot].prec == KUQCD_DOUBLE)
-        return solve_ref_t<double, LinkD>(slot, link_d(c.fat[slot]), link_d(c.lng[slot]), x, b, p);
-    return solve_ref_t<float, LinkF>(slot, link_f(c.fat[slot]), link_f(c.lng[slot]), x, b, p);
+    if (full)
+        return solve_full_t<float, LinkF, LinkF>(slot, link_f(c.fat[slot]), link_f(c.lng[slot]), x, b, p);
+    return solve_ref_t<float, LinkF, LinkF>(slot, link_f(c.fat[slot]), link_f(c.lng[slot]), x, b, p);
 }
 
 // ---------------------------------------------------------------- FP64 verification
@@ -207,8 +214,9 @@ KuQCDStatus solve_reference(int slot, void **x, void *b, KuQCDInvertParam &p)
 // SoA with stride 3Vh, A_k = sigma_k - D D with absolute shifts sigma_k. The
 // FP64 hopping term runs batched over the listed vectors (links read once per
 // pair of vectors).
-static std::vector<double> true_res2(int slot, int parity, const double *b, const double *x, int nk,
-                                     const std::vector<double> &sigma_abs, std::vector<int> which = {})
+template <class LL>
+static std::vector<double> true_res2_t(int slot, LinkD fat, LL lng, int parity, const double *b, const double *x,
+                                       int nk, const std::vector<double> &sigma_abs, std::vector<int> which)
 {
     Context &c = ctx();
     const Geom g = slot_geom(slot);
@@ -231,10 +239,10 @@ static std::vector<double> true_res2(int slot, int parity, const double *b, cons
     KQ_HIP_CHECK(hipMemcpyAsync(sig.as<void>(), sigma_abs.data(), sizeof(double) * nk, hipMemcpyHostToDevice,
                                 c.stream));
     KQ_HIP_CHECK(hipMemcpyAsync(list.as<void>(), hl.data(), sizeof(int) * (nl + 1), hipMemcpyHostToDevice, c.stream));
-    DslashArgs<LinkD, LinkD, double> a{};
+    DslashArgs<LinkD, LL, double> a{};
     a.g = g;
-    a.fat = link_d(c.fat[slot]);
-    a.lng = link_d(c.lng[slot]);
+    a.fat = fat;
+    a.lng = lng;
     a.nrhs = nl;
     a.rlist = list.as<int>();
     a.nact = list.as<int>() + nl;
@@ -259,6 +267,16 @@ static std::vector<double> true_res2(int slot, int parity, const double *b, cons
     for (int k : which)
         out[k] = all[k];
     KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+    return out;
+}
+
+static std::vector<double> true_res2(int slot, int parity, const double *b, const double *x, int nk,
+                                     const std::vector<double> &sigma_abs, std::vector<int> which = {})
+{
+    std::vector<double> out;
+    with_fp64_links(slot, [&](LinkD fat, auto lng) {
+        out = true_res2_t(slot, fat, lng, parity, b, x, nk, sigma_abs, which);
+    });
     return out;
 }
 
@@ -313,7 +331,7 @@ static void refine_pairs(Problem &P, LF fat, LL lng, DevBuf &x_out, const std::v
     else
         tune.nr_by_live = nr_table(P.slot, P.o->mode, nsys);
     CgmRunner<double, float, LF, LL> ref(P.g, P.parity, fat, lng, nsys, nsys, c.stream, tune);
-    ref.enable_reliable(link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.delta);
+    with_fp64_links(P.slot, [&](LinkD f, auto l) { ref.enable_reliable(f, l, P.delta); });
     for (int i = 0; i < nsys; ++i) {
         const int r = need[i].first, k = need[i].second;
         const double sh = abs_shifts(*P.p, r)[k];
@@ -364,7 +382,7 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
     auto t0 = clk::now();
     CgmRunner<double, float, LF, LL> run(P.g, P.parity, fat, lng, max_sys, (size_t)nrhs * ns + (size_t)nrhs * (ns - 1),
                                          c.stream, tune);
-    run.enable_reliable(link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.delta);
+    with_fp64_links(P.slot, [&](LinkD f, auto l) { run.enable_reliable(f, l, P.delta); });
     // the multishift systems fold their solution updates per segment instead of updating every
     // shift's x and p each iteration (KQ_DEFERRED=0 switches this off)
     run.enable_deferred(deferred_slots<float>(nrhs, P.g.Vh));
@@ -502,24 +520,29 @@ static void run_fp64(Problem &P, DevBuf &x_out, KqSolveReport *rep, std::vector<
     set_caps(tune, P.slot, KQ_MODE_FP64, false);
     tune.chunk = P.o->chunk > 0 ? P.o->chunk : 16;
     auto t0 = clk::now();
-    CgmRunner<double, double, LinkD, LinkD> run(P.g, P.parity, link_d(c.fat[P.slot]), link_d(c.lng[P.slot]), P.nrhs,
-                                                (size_t)P.nrhs * P.ns, c.stream, tune);
-    run.enable_deferred(deferred_slots<double>(P.nrhs, P.g.Vh));
-    for (int r = 0; r < P.nrhs; ++r) {
-        std::vector<double> tolk(P.ns, P.tol[r]);
-        run.add_system(P.ns, P.p->shifts + (size_t)r * P.ns, tolk.data(), P.p->max_iter,
-                       P.b->as<double>() + (size_t)r * 2 * n3, nullptr);
-    }
-    run.run();
+    std::vector<int> iters(P.nrhs);
     x_out.alloc(sizeof(double2) * P.nrhs * P.ns * n3);
-    for (int r = 0; r < P.nrhs; ++r)
-        for (int k = 0; k < P.ns; ++k)
-            KQ_HIP_CHECK(hipMemcpyAsync(x_out.as<double>() + ((size_t)r * P.ns + k) * 2 * n3, run.x_lo(r, k),
-                                        sizeof(double2) * n3, hipMemcpyDeviceToDevice, c.stream));
-    KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+    with_fp64_links(P.slot, [&](auto fat, auto lng) {
+        CgmRunner<double, double, decltype(fat), decltype(lng)> run(P.g, P.parity, fat, lng, P.nrhs,
+                                                                     (size_t)P.nrhs * P.ns, c.stream, tune);
+        run.enable_deferred(deferred_slots<double>(P.nrhs, P.g.Vh));
+        for (int r = 0; r < P.nrhs; ++r) {
+            std::vector<double> tolk(P.ns, P.tol[r]);
+            run.add_system(P.ns, P.p->shifts + (size_t)r * P.ns, tolk.data(), P.p->max_iter,
+                           P.b->as<double>() + (size_t)r * 2 * n3, nullptr);
+        }
+        run.run();
+        for (int r = 0; r < P.nrhs; ++r) {
+            iters[r] = run.state(r).iter;
+            for (int k = 0; k < P.ns; ++k)
+                KQ_HIP_CHECK(hipMemcpyAsync(x_out.as<double>() + ((size_t)r * P.ns + k) * 2 * n3, run.x_lo(r, k),
+                                            sizeof(double2) * n3, hipMemcpyDeviceToDevice, c.stream));
+        }
+        KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+    });
     auto t1 = clk::now();
     for (int r = 0; r < P.nrhs; ++r) {
-        rep[r].iter = run.state(r).iter;
+        rep[r].iter = iters[r];
         rep[r].seconds_multishift = secs(t0, t1);
     }
     res2.assign((size_t)P.nrhs * P.ns, 0.0);

@@ -298,11 +298,18 @@ public:
     // Mixed precision: FP64 links for the true residual and FP64 accumulators.
     // A system pauses (waits for its reliable update) once |r|^2 falls below
     // pause_factor * delta^2 * (|r|^2 at the last update).
-    void enable_reliable(LinkD fat64, LinkD lng64, double delta, double pause_factor = 1e-2)
+    // lng64: LinkD, or LinkD14 when the long links have that form
+    template <class LL64> void enable_reliable(LinkD fat64, LL64 lng64, double delta, double pause_factor = 1e-2)
     {
         reliable_ = true;
         fat64_ = fat64;
-        lng64_ = lng64;
+        if constexpr (std::is_same<LL64, LinkD14>::value) {
+            lng64r_ = lng64;
+            l14_ = true;
+        } else {
+            lng64_ = lng64;
+            l14_ = false;
+        }
         delta_ = delta;
         pause2_ = pause_factor * delta * delta;
         b_hi_.alloc(sizeof(double2) * max_sys_ * n3_);
@@ -598,12 +605,19 @@ private:
     // second pass over the links (478 us against 2 x 195 us at 32^4)
     void apply_a64_x0(int n)
     {
+        if (l14_)
+            apply_a64_x0_t(n, lng64r_);
+        else
+            apply_a64_x0_t(n, lng64_);
+    }
+    template <class LL64> void apply_a64_x0_t(int n, LL64 lng64)
+    {
         const int nr = n >= 2 ? 2 : 1;
         const int cap = tune_.cap64[nr - 1];
-        DslashArgs<LinkD, LinkD, double> a{};
+        DslashArgs<LinkD, LL64, double> a{};
         a.g = g_;
         a.fat = fat64_;
-        a.lng = lng64_;
+        a.lng = lng64;
         a.rlist = list();
         a.nact = list_n();
         a.nrhs = n;
@@ -780,6 +794,8 @@ private:
     std::vector<char> finalized_;
     bool reliable_ = false;
     LinkD fat64_{}, lng64_{};
+    LinkD14 lng64r_{};
+    bool l14_ = false;
     double delta_ = 0.1, pause2_ = 0.0;
     DevBuf b_hi_, x_hi_, tmp64_, ax64_;
     int ndslot_ = 0, dslot_used_ = 0;

@@ -111,3 +111,34 @@ def test_dslash_antihermitian_full_size(gpu):
     a = np.vdot(cu, Dv[..., 0] + 1j * Dv[..., 1])
     b = np.vdot(Du[..., 0] + 1j * Du[..., 1], cv)
     assert abs(a + b) <= 1e-12 * abs(a)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant,b_site", [("su3", 2144), ("u3", 2144), ("perturbed", 2400)])
+def test_dslash_fp64_long_link_formats(gpu, variant, b_site):
+    """FP64 long links are kept in 14 reals (two rows of an SU(3) matrix and a complex scale) when
+    every link is a complex multiple of an SU(3) matrix to 1e-13: products of SU(3) links times
+    c_Naik, and the same times a U(1) phase per link (U(3) links, as HISQ's reunitarised ones).
+    Links that are not (a 1e-6 perturbation) keep 18 reals. The operator is the oracle's to 1e-13
+    either way; the algorithmic bytes per site (8 x 144 fat + 8 x 112 or 144 long + 2 x 48)
+    show which format ran."""
+    kq = gpu
+    lat = (8, 8, 8, 8)
+    fat, lng = kq.gen_gauge(lat, kind=kq.KQ_GEN_HAAR, seed_fat=21, seed_long=22)
+    rng = np.random.default_rng(23)
+    if variant == "u3":
+        ph = np.exp(1j * rng.uniform(0, 2 * np.pi, lng.shape[:2]))[..., None]  # (V, 4, 1)
+        z = (lng[..., 0] + 1j * lng[..., 1]) * ph
+        lng = np.ascontiguousarray(np.stack([z.real, z.imag], axis=-1))
+    elif variant == "perturbed":
+        lng = lng + 1e-6 * rng.standard_normal(lng.shape)
+    kq.load_gauge_pair(fat, lng, lat)
+    st, r = kq.bench_dslash(kq.KQ_MODE_FP64, 1, 1, 256, 1)
+    assert st == 0, kq.last_error()
+    assert round(r.bytes / (np.prod(lat) // 2)) == b_site, variant
+    for parity in (0, 1):
+        src = kq.gen_source(lat, 1 - parity, seed=24)
+        dst = np.empty_like(src)
+        assert kq.dslash(src, dst, parity) == 0, kq.last_error()
+        assert rel_err(dst, oracle.dslash(fat, lng, src, lat, parity)) <= 1e-13, (variant, parity)
+    kq.kuqcd_gauge_free()
