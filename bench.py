@@ -47,8 +47,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(path: Path, kernel: str = "k_dslash<kq::LinkD, kq::LinkD, double, 1"):
-    """HBM bytes per launch of a hopping-term kernel (default: the FP64 term) from the committed
+def pmc_traffic(path: Path, kernel: str):
+    """HBM bytes per launch of a hopping-term kernel (name prefix as rocprofv3 prints it) from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
     if not path.exists():
         return None

@@ -22,11 +22,14 @@ def test_multi_gpu_request_exits_with_message():
 def test_traffic_summary_parses():
     import bench
 
-    t = bench.pmc_traffic(ROOT / "profiles" / "dslash_32x32x32x32_pmc.txt")
+    t = bench.pmc_traffic(ROOT / "profiles" / "dslash_32x32x32x32_pmc.txt", "k_dslash<kq::LinkD, kq::LinkD14, double, 1")
     assert t is not None
-    # the FP64 term's compulsory bytes are 2400 B/site x 524288 sites = 1.258 GB; the counters
-    # may only add to that (neighbour re-reads), and not by more than a few percent
-    assert 1.258e9 <= t <= 1.4e9
+    # the FP64 term's compulsory bytes (14-real long links) are 2144 B/site x 524288 sites = 1.124 GB;
+    # the counters may only add to that (neighbour re-reads), and not by more than a few percent
+    assert 1.124e9 <= t <= 1.25e9
+    # the 4-right-hand-side FP32 term: 1184 B/site = 0.621 GB compulsory, 41 % more fetched
+    t4 = bench.pmc_traffic(ROOT / "profiles" / "dslash_32x32x32x32_pmc.txt", "k_dslash<kq::LinkF, kq::LinkF12, float, 4")
+    assert t4 is not None and 0.621e9 <= t4 <= 1.0e9
 
 
 @pytest.mark.parametrize("name", ["dslash_32x32x32x32_pmc.txt", "dslash_32x32x32x32_kernels_only_stats.txt",
