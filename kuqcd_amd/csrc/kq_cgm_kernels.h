@@ -120,8 +120,10 @@ __host__ __device__ inline void seg_coefficient(const double *h, int J, int k, i
 template <typename S>
 void seg_coef(const CgmState<S> *st, const double *shist, double *coef, const int *list, const int *n, int nsys,
               hipStream_t stream);
-template <typename S, typename VR>
-void seg_fold(const CgmState<S> *st, const double *coef, double *x_hi, VR *p, const VR *p0snap, const VR *rhist,
+// XR: real type of the solutions the folds add to (double: x_hi, or the FP64 solutions; float:
+// the FP32 solutions of the float reference method)
+template <typename S, typename VR, typename XR>
+void seg_fold(const CgmState<S> *st, const double *coef, XR *x, VR *p, const VR *p0snap, const VR *rhist,
               const size_t *off, uint32_t Vh, const int *list, const int *n, int nsys, int kmax, hipStream_t stream);
 template <typename S, typename VR>
 void seg_reset(CgmState<S> *st, const VR *p, VR *p0snap, const size_t *off, uint32_t Vh, const int *list, const int *n,

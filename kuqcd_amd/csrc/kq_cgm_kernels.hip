@@ -524,12 +524,13 @@ void seg_coef(const CgmState<S> *st, const double *shist, double *coef, const in
 // (linear id % 8), so blockIdx.x = 8 (k + kmax * (y / 8)) + y % 8 for shift k and site range y:
 // the kmax workgroups of one site range run back to back on one XCD and read its recorded
 // residuals once into that XCD's L2.
-template <typename S, typename VR>
-__global__ void __launch_bounds__(kBlasNT) k_seg_fold(const CgmState<S> *st, const double *coef, double *x_hi, VR *p,
+template <typename S, typename VR, typename XR>
+__global__ void __launch_bounds__(kBlasNT) k_seg_fold(const CgmState<S> *st, const double *coef, XR *x, VR *p,
                                                       const VR *p0snap, const VR *rhist, const size_t *off,
                                                       uint32_t Vh, const int *list, const int *n, int kmax)
 {
     using V2 = c2_t<VR>;
+    using X2 = c2_t<XR>;
     const int slot = blockIdx.z;
     const uint32_t rest = blockIdx.x / 8;
     const int k = (int)(rest % (uint32_t)kmax);
@@ -564,10 +565,10 @@ __global__ void __launch_bounds__(kBlasNT) k_seg_fold(const CgmState<S> *st, con
         pi += cf[kHistMax + 2 + j] * (double)rv.y;
     }
     if (keep_x) {
-        double2 *xh = reinterpret_cast<double2 *>(x_hi) + pe;
-        double2 h = ld_nt(xh);
-        h.x += xr;
-        h.y += xi;
+        X2 *xh = reinterpret_cast<X2 *>(x) + pe;
+        X2 h = ld_nt(xh);
+        h.x = (XR)((double)h.x + xr);
+        h.y = (XR)((double)h.y + xi);
         // the unshifted solution is read next by the FP64 hopping term of a reliable update
         if (k == 0)
             *xh = h;
@@ -582,8 +583,8 @@ __global__ void __launch_bounds__(kBlasNT) k_seg_fold(const CgmState<S> *st, con
     }
 }
 
-template <typename S, typename VR>
-void seg_fold(const CgmState<S> *st, const double *coef, double *x_hi, VR *p, const VR *p0snap, const VR *rhist,
+template <typename S, typename VR, typename XR>
+void seg_fold(const CgmState<S> *st, const double *coef, XR *x, VR *p, const VR *p0snap, const VR *rhist,
               const size_t *off, uint32_t Vh, const int *list, const int *n, int nsys, int kmax, hipStream_t stream)
 {
     if (kmax <= 0)
@@ -593,7 +594,7 @@ void seg_fold(const CgmState<S> *st, const double *coef, double *x_hi, VR *p, co
     const size_t gx = 8 * (size_t)kmax * nb8;
     if (kmax > kMaxShift || gx > 0x7fffffffu)
         throw std::invalid_argument("seg_fold: grid too large");
-    hipLaunchKernelGGL((k_seg_fold<S, VR>), dim3((unsigned)gx, 1, nsys), dim3(kBlasNT), 0, stream, st, coef, x_hi, p,
+    hipLaunchKernelGGL((k_seg_fold<S, VR, XR>), dim3((unsigned)gx, 1, nsys), dim3(kBlasNT), 0, stream, st, coef, x, p,
                        p0snap, rhist, off, Vh, list, n, kmax);
     KQ_HIP_CHECK(hipGetLastError());
 }
@@ -736,13 +737,18 @@ KQ_INST_S(float)
                                   const int *, int, hipStream_t);                                             \
     template void copy_r_to_p<S, VR>(const CgmState<S> *, Vec<VR>, VR *, const size_t *, const int *,          \
                                      const int *, int, hipStream_t);                                          \
-    template void seg_fold<S, VR>(const CgmState<S> *, const double *, double *, VR *, const VR *, const VR *,  \
-                                  const size_t *, uint32_t, const int *, const int *, int, int, hipStream_t);  \
+    template void seg_fold<S, VR, double>(const CgmState<S> *, const double *, double *, VR *, const VR *,    \
+                                          const VR *, const size_t *, uint32_t, const int *, const int *, int, int, \
+                                          hipStream_t);                                                        \
     template void seg_reset<S, VR>(CgmState<S> *, const VR *, VR *, const size_t *, uint32_t, const int *,     \
                                    const int *, int, hipStream_t);
 KQ_INST_SV(double, double)
 KQ_INST_SV(float, float)
 KQ_INST_SV(double, float)
+// the float reference method folds into its FP32 solutions
+template void seg_fold<float, float, float>(const CgmState<float> *, const double *, float *, float *, const float *,
+                                            const float *, const size_t *, uint32_t, const int *, const int *, int, int,
+                                            hipStream_t);
 
 template void rel_residual<float>(const double *, const double *, Vec<float>, const int *, const int *, int,
                                   double *, uint32_t, hipStream_t);

@@ -461,8 +461,7 @@ int CGM<T, VecT>::Invert(int noShifts, T shifts[], const VecT &src, VecT *sol[],
         tune.nt = kq::kReproNT;
         tune.chunk = 16;
         kq::CgmRunner<T, T, LK, LK> run(g, parity, f, l, 1, noShifts, st, tune);
-        if constexpr (std::is_same<T, double>::value)
-            run.enable_deferred(kq::deferred_slots<double>(1, g.Vh));
+        run.enable_deferred(kq::deferred_slots<T>(1, g.Vh));
         const std::vector<double> tolk(noShifts, tol);
         run.add_system(noShifts, shifts, tolk.data(), maxIters, src.DevicePtr(), nullptr);
         run.run();

@@ -76,10 +76,9 @@ static KuQCDStatus solve_ref_t(int slot, LF fat, LL lng, void **x, void *b, KuQC
     set_caps(tune, slot, KQ_MODE_REFERENCE, false);
     tune.chunk = 16;
     CgmRunner<T, T, LF, LL> run(g, parity, fat, lng, 1, ns, c.stream, tune);
-    // FP64 solutions take their updates per segment of recorded residuals: the same iterations
+    // the solutions take their updates per segment of recorded residuals: the same iterations
     // and scalars as updating every shift each iteration, with a quarter of the vector traffic
-    if constexpr (std::is_same<T, double>::value)
-        run.enable_deferred(deferred_slots<double>(1, g.Vh));
+    run.enable_deferred(deferred_slots<T>(1, g.Vh));
     DevBuf stage, bdev(sizeof(c2_t<T>) * 3 * (size_t)g.Vh);
     const size_t n3 = (size_t)3 * g.Vh;
     upload_vec<T, T>(b, Vec<T>{bdev.as<c2_t<T>>(), g.Vh, n3}, 0, stage, c.stream);

@@ -320,12 +320,13 @@ public:
 
     // Deferred solution updates for up to `slots` multishift systems: each records its residuals
     // and recurrence scalars, keeps only p_0 current, and has its x_k and p_k brought up to date
-    // by segment folds (kq_cgm_kernels.h). The folds accumulate in FP64: into x_hi in the
-    // reliable mode, into the solutions themselves when they are FP64 (VR = double).
+    // by segment folds (kq_cgm_kernels.h). The folds form each segment's update in FP64 and add
+    // it to x_hi in the reliable mode, else to the solutions themselves (VR = S: the FP64 solve
+    // and the reference method in double or float).
     void enable_deferred(int slots)
     {
-        if (!reliable_ && !std::is_same<VR, double>::value)
-            throw std::logic_error("deferred solution updates need FP64 solutions or the reliable mode");
+        if (!reliable_ && !std::is_same<S, VR>::value)
+            throw std::logic_error("deferred solution updates: the solutions are x_hi or of the scalar type");
         if (tune_.chunk > kHistMax)
             throw std::invalid_argument("deferred solution updates: chunk longer than the history");
         ndslot_ = std::max(0, slots);
@@ -698,10 +699,17 @@ private:
                 set_list(seg);
                 CgmState<S> *stp = state_.as<CgmState<S>>();
                 seg_coef<S>(stp, shist_.as<double>(), coef_.as<double>(), list(), list_n(), list_grid(), st_);
-                // the folds accumulate into the FP64 solutions: x_hi, or x_lo itself when VR is double
-                double *xf = reliable_ ? x_hi_.as<double>() : x_.as<double>();
-                seg_fold<S, VR>(stp, coef_.as<double>(), xf, p_.as<VR>(), p0snap_.as<VR>(), rhist_.as<VR>(),
-                                off_.as<size_t>(), g_.Vh, list(), list_n(), list_grid(), kmax, st_);
+                // the folds accumulate into x_hi (FP64) in the reliable mode, else into the solutions
+                // themselves (FP64 solve, reference method in either precision)
+                if (reliable_) {
+                    seg_fold<S, VR, double>(stp, coef_.as<double>(), x_hi_.as<double>(), p_.as<VR>(), p0snap_.as<VR>(),
+                                            rhist_.as<VR>(), off_.as<size_t>(), g_.Vh, list(), list_n(), list_grid(),
+                                            kmax, st_);
+                } else if constexpr (std::is_same<S, VR>::value) {
+                    seg_fold<S, VR, VR>(stp, coef_.as<double>(), x_.as<VR>(), p_.as<VR>(), p0snap_.as<VR>(),
+                                        rhist_.as<VR>(), off_.as<size_t>(), g_.Vh, list(), list_n(), list_grid(), kmax,
+                                        st_);
+                }
                 seg_reset<S, VR>(stp, p_.as<VR>(), p0snap_.as<VR>(), off_.as<size_t>(), g_.Vh, list(), list_n(),
                                  list_grid(), st_);
             }
