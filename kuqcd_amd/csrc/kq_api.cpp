@@ -88,7 +88,12 @@ LinkD link_d(const GaugeSlot &s)
 LinkD14 link_d14(const GaugeSlot &s)
 {
     const uint32_t Vh = (uint32_t)((uint64_t)s.lat[0] * s.lat[1] * s.lat[2] * s.lat[3] / 2);
-    return LinkD14{s.kf14.as<char>(), nb64_of(Vh)};
+    return LinkD14{s.kfr.as<char>(), nb64_of(Vh)};
+}
+LinkF12 link_f12(const GaugeSlot &s)
+{
+    const uint32_t Vh = (uint32_t)((uint64_t)s.lat[0] * s.lat[1] * s.lat[2] * s.lat[3] / 2);
+    return LinkF12{s.kfr.as<char>(), nb64_of(Vh)};
 }
 
 LinkF link_f(const GaugeSlot &s)
@@ -234,22 +239,28 @@ static void upload_links(const HR *h, GaugeSlot &slot, double scale, int prec, b
     if (prec == KUQCD_DOUBLE) {
         slot.kf.alloc(link_field_bytes<LinkD>(Vh));
         canon_to_linkD(slot.soa.as<double2>(), slot.kf.as<char>(), Vh, c.stream);
-        // long links also as 14 reals when every one is a complex multiple of an SU(3) matrix
-        if (long_links) {
-            DevBuf fail(sizeof(int));
-            KQ_HIP_CHECK(hipMemsetAsync(fail.as<void>(), 0, sizeof(int), c.stream));
-            slot.kf14.alloc(link_field_bytes<LinkD14>(Vh));
-            canon_to_linkD14(slot.soa.as<double2>(), slot.kf14.as<char>(), Vh, fail.as<int>(), c.stream);
-            int nfail = 0;
-            KQ_HIP_CHECK(hipMemcpyAsync(&nfail, fail.as<void>(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
-            KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
-            slot.d14 = (nfail == 0);
-            if (!slot.d14)
-                slot.kf14.release();
-        }
     } else {
         slot.kf.alloc(link_field_bytes<LinkF>(Vh));
         canon_to_linkF<float>(slot.soa.as<float2>(), slot.kf.as<char>(), Vh, c.stream);
+    }
+    // long links also in fewer reals when every one has that form: 14 in double (a complex
+    // multiple of an SU(3) matrix, to 1e-13), 12 in float (a real multiple, to 1e-6)
+    if (long_links) {
+        DevBuf fail(sizeof(int));
+        KQ_HIP_CHECK(hipMemsetAsync(fail.as<void>(), 0, sizeof(int), c.stream));
+        if (prec == KUQCD_DOUBLE) {
+            slot.kfr.alloc(link_field_bytes<LinkD14>(Vh));
+            canon_to_linkD14(slot.soa.as<double2>(), slot.kfr.as<char>(), Vh, fail.as<int>(), c.stream);
+        } else {
+            slot.kfr.alloc(link_field_bytes<LinkF12>(Vh));
+            canon_to_linkF12(slot.soa.as<float2>(), slot.kfr.as<char>(), Vh, fail.as<int>(), c.stream);
+        }
+        int nfail = 0;
+        KQ_HIP_CHECK(hipMemcpyAsync(&nfail, fail.as<void>(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+        slot.lrecon = (nfail == 0);
+        if (!slot.lrecon)
+            slot.kfr.release();
     }
     KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
 }
@@ -325,8 +336,8 @@ static KuQCDStatus load_gauge(void *h_gauge, KuQCDGaugeParam *param)
     gs.loaded = false;
     gs.soa.release();
     gs.kf.release();
-    gs.kf14.release();
-    gs.d14 = false;
+    gs.kfr.release();
+    gs.lrecon = false;
     free_derived(slot);
     for (int i = 0; i < 4; ++i)
         gs.lat[i] = param->lat[i];
@@ -336,7 +347,8 @@ static KuQCDStatus load_gauge(void *h_gauge, KuQCDGaugeParam *param)
         upload_links<double>(static_cast<const double *>(h_gauge), gs, scale, KUQCD_DOUBLE,
                              param->link_type == KUQCD_LONG_LINK);
     else
-        upload_links<float>(static_cast<const float *>(h_gauge), gs, scale, KUQCD_SINGLE, false);
+        upload_links<float>(static_cast<const float *>(h_gauge), gs, scale, KUQCD_SINGLE,
+                            param->link_type == KUQCD_LONG_LINK);
     gs.loaded = true;
     return KUQCD_SUCCESS;
 }

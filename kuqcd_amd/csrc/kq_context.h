@@ -24,8 +24,10 @@ struct GaugeSlot {
     unsigned lat[4] = {0, 0, 0, 0};
     DevBuf soa;  // canonical, double2 or float2
     DevBuf kf;   // LinkD (double) or LinkF (float) format
-    bool d14 = false;  // double long links also held as LinkD14 in kf14
-    DevBuf kf14;
+    // long links also held in fewer reals in kfr: LinkD14 (double), LinkF12 (float), when every
+    // link has that form
+    bool lrecon = false;
+    DevBuf kfr;
 };
 
 // Lower-precision kernel formats of a double gauge pair, made on first use.
@@ -72,8 +74,9 @@ Geom slot_geom(int slot);
 
 LinkD link_d(const GaugeSlot &s);
 LinkD14 link_d14(const GaugeSlot &s);
+LinkF12 link_f12(const GaugeSlot &s);
 // fn(LinkD fat, lng) with the long links of the slot's double pair as LinkD14 when every one
-// has that form (GaugeSlot::d14), else as LinkD: the FP64 hopping term of the solvers, the FP64
+// has that form (GaugeSlot::lrecon), else as LinkD: the FP64 hopping term of the solvers, the FP64
 // verification and the operator entry points
 template <class Fn> inline void with_fp64_links(int slot, Fn &&fn);
 LinkF link_f(const GaugeSlot &s);
@@ -122,10 +125,21 @@ template <class Fn> bool with_mixed_links(int slot, int mode, Fn &&fn)
 template <class Fn> inline void with_fp64_links(int slot, Fn &&fn)
 {
     Context &c = ctx();
-    if (c.lng[slot].d14)
+    if (c.lng[slot].lrecon)
         fn(link_d(c.fat[slot]), link_d14(c.lng[slot]));
     else
         fn(link_d(c.fat[slot]), link_d(c.lng[slot]));
+}
+
+// fn(LinkF fat, lng) for a float pair: long links as LinkF12 when every one is a real multiple of
+// an SU(3) matrix (GaugeSlot::lrecon), else as LinkF (the float reference method and operators)
+template <class Fn> inline void with_fp32_links(int slot, Fn &&fn)
+{
+    Context &c = ctx();
+    if (c.lng[slot].lrecon)
+        fn(link_f(c.fat[slot]), link_f12(c.lng[slot]));
+    else
+        fn(link_f(c.fat[slot]), link_f(c.lng[slot]));
 }
 
 // device address of the no-allocation entry point's static scratch array on the current

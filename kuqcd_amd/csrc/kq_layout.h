@@ -32,6 +32,7 @@ void canon_to_linkH(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st
 void canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);
 // LinkF12 from canonical double links; *fail counts links that are not b * SU(3)
 void canon_to_linkF12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);
+void canon_to_linkF12(const float2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);
 
 // host-layout colour vector (device copy) <-> SoA vector of right-hand side r
 template <typename HR, typename VR>

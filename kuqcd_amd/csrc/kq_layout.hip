@@ -143,13 +143,16 @@ __device__ inline double2 det3(const double2 m[9])
     return add(add(mul(m[0], c0), mul(m[1], c1)), mul(m[2], c2));
 }
 
-// V = U / b with b = cbrt(Re det U), checked to be unitary; false if U is not b * SU(3)
-__device__ inline bool su3_factor(const double2 *canon, uint32_t qm, uint32_t s, uint32_t Vh, double2 v[9],
-                                  double &bsc)
+// V = U / b with b = cbrt(Re det U), checked to be unitary; false if U is not b * SU(3).
+// C2: double2 or float2 canonical links (formed in double either way)
+template <typename C2>
+__device__ inline bool su3_factor(const C2 *canon, uint32_t qm, uint32_t s, uint32_t Vh, double2 v[9], double &bsc)
 {
     double2 m[9];
-    for (int k = 0; k < 9; ++k)
-        m[k] = canon[((size_t)qm * 9 + k) * Vh + s];
+    for (int k = 0; k < 9; ++k) {
+        const C2 c = canon[((size_t)qm * 9 + k) * Vh + s];
+        m[k] = mk<double2>((double)c.x, (double)c.y);
+    }
     const double2 d = det3(m);
     // b^3 = det(b V) must be real; b = cbrt(Re det)
     bsc = cbrt(d.x);
@@ -259,7 +262,8 @@ void canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh, int *fail, h
     KQ_HIP_CHECK(hipGetLastError());
 }
 
-__global__ void k_canon_to_linkF12(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64, int *fail)
+template <typename C2>
+__global__ void k_canon_to_linkF12(const C2 *canon, char *dst, uint32_t Vh, uint32_t nb64, int *fail)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)8 * Vh)
@@ -277,14 +281,24 @@ __global__ void k_canon_to_linkF12(const double2 *canon, char *dst, uint32_t Vh,
     reinterpret_cast<float *>(b + 3072)[l] = (float)bsc;
 }
 
-void canon_to_linkF12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+template <typename C2> static void canon_to_linkF12_t(const C2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
 {
     const size_t n = (size_t)8 * Vh;
     if (n == 0)
         return;
-    hipLaunchKernelGGL(k_canon_to_linkF12, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
+    hipLaunchKernelGGL(k_canon_to_linkF12<C2>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
                        nb64_of(Vh), fail);
     KQ_HIP_CHECK(hipGetLastError());
+}
+
+void canon_to_linkF12(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+{
+    canon_to_linkF12_t(canon, dst, Vh, fail, st);
+}
+
+void canon_to_linkF12(const float2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+{
+    canon_to_linkF12_t(canon, dst, Vh, fail, st);
 }
 
 template <typename HR, typename VR>

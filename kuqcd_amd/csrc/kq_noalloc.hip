@@ -162,9 +162,10 @@ extern "C" KuQCDStatus kq_mdaggm_noalloc(const void *src, void *dst, int parity,
                                                                      c.stream);
             });
         else
-            mdaggm_noalloc<LinkF, LinkF, float>(link_f(c.fat[slot]), link_f(c.lng[slot]), g,
-                                                static_cast<const float *>(src), static_cast<float *>(dst), parity,
-                                                sigma, c.stream);
+            with_fp32_links(slot, [&](auto fat, auto lng) {
+                mdaggm_noalloc<decltype(fat), decltype(lng), float>(fat, lng, g, static_cast<const float *>(src),
+                                                                    static_cast<float *>(dst), parity, sigma, c.stream);
+            });
         return KUQCD_SUCCESS;
     } catch (const std::exception &e) {
         set_error(e.what());

@@ -98,8 +98,10 @@ static KuQCDStatus apply(int nrhs, const void *src, void *dst, int parity, int m
                                                                       mdaggm, sigma);
             });
         else if (mode == KQ_MODE_REFERENCE)
-            apply_t<LinkF, LinkF, float, float>(link_f(c.fat[slot]), link_f(c.lng[slot]), g, nrhs, src, dst, parity,
-                                                nr, nt, mdaggm, sigma);
+            with_fp32_links(slot, [&](auto fat, auto lng) {
+                apply_t<decltype(fat), decltype(lng), float, float>(fat, lng, g, nrhs, src, dst, parity, nr, nt, mdaggm,
+                                                                    sigma);
+            });
         else {
             set_error("FP64 mode needs a double-precision gauge field");
             return KUQCD_ERR_UNSUPPORTED;
@@ -168,7 +170,7 @@ static double site_bytes(int mode, bool dbl_gauge, bool long12, int nrhs)
         bv = 24;
     } else if (mode == KQ_MODE_MIXED_FP32 || !dbl_gauge) {
         bl_f = 72;
-        bl_l = (long12 && mode == KQ_MODE_MIXED_FP32) ? 48 + 4 : 72;
+        bl_l = long12 ? 48 + 4 : 72;  // LinkF12 (mixed FP32 mode, or a float pair's long links)
         bv = 24;
     } else {
         bl_f = 144;
@@ -188,13 +190,18 @@ static double time_mode(int slot, int mode, int nrhs, int nr, int nt, int iters,
     if (mode == KQ_MODE_REFERENCE || mode == KQ_MODE_FP64) {
         if (dbl) {
             double ms = 0.0;
-            *long12 = c.lng[slot].d14;
+            *long12 = c.lng[slot].lrecon;
             with_fp64_links(slot, [&](auto fat, auto lng) {
                 ms = time_dslash<decltype(fat), decltype(lng), double>(fat, lng, g, nrhs, nr, nt, iters, cap);
             });
             return ms;
         }
-        return time_dslash<LinkF, LinkF, float>(link_f(c.fat[slot]), link_f(c.lng[slot]), g, nrhs, nr, nt, iters, cap);
+        double ms = 0.0;
+        *long12 = c.lng[slot].lrecon;
+        with_fp32_links(slot, [&](auto fat, auto lng) {
+            ms = time_dslash<decltype(fat), decltype(lng), float>(fat, lng, g, nrhs, nr, nt, iters, cap);
+        });
+        return ms;
     }
     double ms = 0.0;
     *long12 = with_mixed_links(slot, mode, [&](auto fat, auto lng) {

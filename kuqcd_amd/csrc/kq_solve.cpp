@@ -203,9 +203,13 @@ KuQCDStatus solve_reference(int slot, void **x, void *b, KuQCDInvertParam &p)
         });
         return st;
     }
-    if (full)
-        return solve_full_t<float, LinkF, LinkF>(slot, link_f(c.fat[slot]), link_f(c.lng[slot]), x, b, p);
-    return solve_ref_t<float, LinkF, LinkF>(slot, link_f(c.fat[slot]), link_f(c.lng[slot]), x, b, p);
+    KuQCDStatus st = KUQCD_SUCCESS;
+    with_fp32_links(slot, [&](auto fat, auto lng) {
+        using LF = decltype(fat);
+        using LL = decltype(lng);
+        st = full ? solve_full_t<float, LF, LL>(slot, fat, lng, x, b, p) : solve_ref_t<float, LF, LL>(slot, fat, lng, x, b, p);
+    });
+    return st;
 }
 
 // ---------------------------------------------------------------- FP64 verification

@@ -142,3 +142,33 @@ def test_dslash_fp64_long_link_formats(gpu, variant, b_site):
         assert kq.dslash(src, dst, parity) == 0, kq.last_error()
         assert rel_err(dst, oracle.dslash(fat, lng, src, lat, parity)) <= 1e-13, (variant, parity)
     kq.kuqcd_gauge_free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant,b_site", [("su3", 1040), ("u3", 1200), ("perturbed", 1200)])
+def test_dslash_fp32_long_link_formats(gpu, variant, b_site):
+    """A float pair's long links are kept in 12 reals (two rows of an SU(3) matrix and a real
+    scale) when every link is a real multiple of an SU(3) matrix to 1e-6; U(3) links with a phase
+    in their determinant and perturbed links keep 18 reals. The operator is the oracle's float
+    operator to 1e-5 either way (8 x 72 fat + 8 x 52 or 72 long + 2 x 24 bytes per site)."""
+    kq = gpu
+    lat = (8, 8, 8, 8)
+    fat, lng = kq.gen_gauge(lat, kind=kq.KQ_GEN_HAAR, seed_fat=31, seed_long=32)
+    rng = np.random.default_rng(33)
+    if variant == "u3":
+        ph = np.exp(1j * rng.uniform(0, 2 * np.pi, lng.shape[:2]))[..., None]
+        z = (lng[..., 0] + 1j * lng[..., 1]) * ph
+        lng = np.ascontiguousarray(np.stack([z.real, z.imag], axis=-1))
+    elif variant == "perturbed":
+        lng = lng + 1e-4 * rng.standard_normal(lng.shape)
+    fat32, lng32 = fat.astype(np.float32), lng.astype(np.float32)
+    kq.load_gauge_pair(fat32, lng32, lat, prec=kq.KUQCD_SINGLE)
+    st, r = kq.bench_dslash(kq.KQ_MODE_REFERENCE, 1, 1, 256, 1)
+    assert st == 0, kq.last_error()
+    assert round(r.bytes / (np.prod(lat) // 2)) == b_site, variant
+    for parity in (0, 1):
+        src = kq.gen_source(lat, 1 - parity, seed=34).astype(np.float32)
+        dst = np.empty_like(src)
+        assert kq.dslash(src, dst, parity) == 0, kq.last_error()
+        assert rel_err(dst, oracle.dslash(fat32, lng32, src, lat, parity)) <= 1e-5, (variant, parity)
+    kq.kuqcd_gauge_free()
