@@ -284,12 +284,14 @@ public:
     void Dslash(const VecT &src, VecT &dst, bool update = false) override;
     void MdaggM(const VecT &src, VecT &dst, bool update = false) override;
 
-    // the links in the kernel format, pre-multiplied by 1/2 (fat) and c3000 (long)
-    const void *KernelLinks(int which) const { return which == 0 ? fat_ : lng_; }
+    // the links in the kernel format, pre-multiplied by 1/2 (fat) and c3000 (long): which = 0 fat,
+    // 1 long (18 reals), 2 long in fewer reals (14 in double, 12 in float; null when some long
+    // link is not a multiple of an SU(3) matrix)
+    const void *KernelLinks(int which) const { return which == 0 ? fat_ : which == 1 ? lng_ : lngr_; }
 
 private:
-    void *fat_ = nullptr, *lng_ = nullptr;
-    size_t link_bytes_ = 0;
+    void *fat_ = nullptr, *lng_ = nullptr, *lngr_ = nullptr;
+    size_t link_bytes_ = 0, lngr_bytes_ = 0;
     VecT *own_tmp_ = nullptr;
     LattBase *own_tmp_base_ = nullptr;
 };

@@ -63,19 +63,32 @@ void check_lattice(const LattBase &lb, size_t real_bytes, unsigned nbatch, const
         throw std::invalid_argument(std::string(what) + ": LattBase gauge order differs from the field's Nbatch");
 }
 
+// kernel link formats of T: type (18 reals) and rtype, the long links in fewer reals (LinkD14:
+// a complex multiple of an SU(3) matrix; LinkF12: a real multiple), whose packer counts in *fail
+// the links without that form
 template <typename T> struct link_fmt;
 template <> struct link_fmt<double> {
     using type = kq::LinkD;
+    using rtype = kq::LinkD14;
     static void make(const double2 *canon, char *dst, uint32_t Vh, hipStream_t st)
     {
         kq::canon_to_linkD(canon, dst, Vh, st);
     }
+    static void make_r(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+    {
+        kq::canon_to_linkD14(canon, dst, Vh, fail, st);
+    }
 };
 template <> struct link_fmt<float> {
     using type = kq::LinkF;
+    using rtype = kq::LinkF12;
     static void make(const float2 *canon, char *dst, uint32_t Vh, hipStream_t st)
     {
         kq::canon_to_linkF<float>(canon, dst, Vh, st);
+    }
+    static void make_r(const float2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+    {
+        kq::canon_to_linkF12(canon, dst, Vh, fail, st);
     }
 };
 
@@ -308,7 +321,21 @@ DiracHISQ<T, NB, RS, RN>::DiracHISQ(KuQCDDslashType diracType, const LinkST &fat
     kq::flat_axpby<T>(static_cast<const kq::c2_t<T> *>(lng.DevicePtr()), scaled.as<kq::c2_t<T>>(), ncomp, c3000, T(0),
                       st);
     link_fmt<T>::make(scaled.as<kq::c2_t<T>>(), static_cast<char *>(lng_), Vh, st);
+    // the long links also in fewer reals when every one has that form (the operator and the
+    // device CGM then read those)
+    using RK = typename link_fmt<T>::rtype;
+    lngr_bytes_ = kq::link_field_bytes<RK>(Vh);
+    lngr_ = take(lngr_bytes_);
+    kq::DevBuf fail(sizeof(int));
+    KQ_HIP_CHECK(hipMemsetAsync(fail.as<void>(), 0, sizeof(int), st));
+    link_fmt<T>::make_r(scaled.as<kq::c2_t<T>>(), static_cast<char *>(lngr_), Vh, fail.as<int>(), st);
+    int nfail = 0;
+    KQ_HIP_CHECK(hipMemcpyAsync(&nfail, fail.as<void>(), sizeof(int), hipMemcpyDeviceToHost, st));
     KQ_HIP_CHECK(hipStreamSynchronize(st));
+    if (nfail != 0) {
+        give(lngr_, lngr_bytes_);
+        lngr_ = nullptr;
+    }
 }
 
 template <typename T, int NB, KuQCDReconstructType RS, KuQCDReconstructType RN> DiracHISQ<T, NB, RS, RN>::~DiracHISQ()
@@ -317,13 +344,14 @@ template <typename T, int NB, KuQCDReconstructType RS, KuQCDReconstructType RN> 
     delete own_tmp_base_;
     give(fat_, link_bytes_);
     give(lng_, link_bytes_);
+    give(lngr_, lngr_bytes_);
 }
 
 // dst (parity q) = D src (parity 1 - q) for the half-lattice blocks at these pointers
-template <typename T, class LK>
-static void hop(const kq::Geom &g, LK fat, LK lng, const void *src, void *dst, int q, hipStream_t st)
+template <typename T, class LF, class LL>
+static void hop(const kq::Geom &g, LF fat, LL lng, const void *src, void *dst, int q, hipStream_t st)
 {
-    kq::DslashArgs<LK, LK, T> a{};
+    kq::DslashArgs<LF, LL, T> a{};
     a.g = g;
     a.dst_parity = q;
     a.fat = fat;
@@ -339,11 +367,20 @@ void DiracHISQ<T, NB, RS, RN>::Dslash(const VecT &src, VecT &dst, bool update)
 {
     (void)update;  // halo exchange only (dirac_hisq.cpp:412, 452); one rank has no halo
     using LK = typename link_fmt<T>::type;
+    using RK = typename link_fmt<T>::rtype;
     const kq::Geom g = geom_of(latBase);
     if (src.latBase.lat.V != latBase.lat.V || dst.latBase.lat.V != latBase.lat.V)
         throw std::invalid_argument("Dslash: fields on another lattice than the links");
     const LK f{static_cast<char *>(fat_), kq::nb64_of(g.Vh)}, l{static_cast<char *>(lng_), kq::nb64_of(g.Vh)};
+    const RK lr{static_cast<char *>(lngr_), kq::nb64_of(g.Vh)};
     hipStream_t st = lib_stream();
+    // dst (parity q) = D src (parity 1 - q), reading the long links in fewer reals when held
+    auto hop = [&](const void *s, void *d, int q) {
+        if (lngr_ != nullptr)
+            ::kuqcd::hop<T>(g, f, lr, s, d, q, st);
+        else
+            ::kuqcd::hop<T>(g, f, l, s, d, q, st);
+    };
     const int qd = parity_of(dst.latBase.site_subset), qs = parity_of(src.latBase.site_subset);
     if (qd >= 0) {
         // dirac_hisq.cpp:415-419: the output parity is dst's; src holds the other one (or both)
@@ -352,7 +389,7 @@ void DiracHISQ<T, NB, RS, RN>::Dslash(const VecT &src, VecT &dst, bool update)
             s += sizeof(kq::c2_t<T>) * 3 * (size_t)g.Vh * (1 - qd);
         else if (qs != 1 - qd)
             throw std::invalid_argument("Dslash: src must hold the parity opposite to dst's");
-        hop<T>(g, f, l, s, dst.DevicePtr(), qd, st);
+        hop(s, dst.DevicePtr(), qd);
         return;
     }
     if (this->diracType == KUQCD_HISQPC_DIRAC)
@@ -363,8 +400,8 @@ void DiracHISQ<T, NB, RS, RN>::Dslash(const VecT &src, VecT &dst, bool update)
     const size_t half = sizeof(kq::c2_t<T>) * 3 * (size_t)g.Vh;
     const char *s = static_cast<const char *>(src.DevicePtr());
     char *d = static_cast<char *>(dst.DevicePtr());
-    hop<T>(g, f, l, s + half, d, 0, st);
-    hop<T>(g, f, l, s, d + half, 1, st);
+    hop(s + half, d, 0);
+    hop(s, d + half, 1);
 }
 
 template <typename T, int NB, KuQCDReconstructType RS, KuQCDReconstructType RN>
@@ -454,22 +491,28 @@ int CGM<T, VecT>::Invert(int noShifts, T shifts[], const VecT &src, VecT *sol[],
         // the device runner of kuqcd_solve_multishift (one system, scalars in T)
         hipStream_t st = lib_stream();
         const kq::Geom g = geom_of(src.latBase);
+        using RK = typename link_fmt<T>::rtype;
         const LK f{static_cast<char *>(const_cast<void *>(h->KernelLinks(0))), kq::nb64_of(g.Vh)};
         const LK l{static_cast<char *>(const_cast<void *>(h->KernelLinks(1))), kq::nb64_of(g.Vh)};
+        const RK lr{static_cast<char *>(const_cast<void *>(h->KernelLinks(2))), kq::nb64_of(g.Vh)};
         kq::RunnerTuning tune;
         tune.nr = 1;
         tune.nt = kq::kReproNT;
         tune.chunk = 16;
-        kq::CgmRunner<T, T, LK, LK> run(g, parity, f, l, 1, noShifts, st, tune);
-        run.enable_deferred(kq::deferred_slots<T>(1, g.Vh));
-        const std::vector<double> tolk(noShifts, tol);
-        run.add_system(noShifts, shifts, tolk.data(), maxIters, src.DevicePtr(), nullptr);
-        run.run();
-        iter = run.state(0).iter;
-        const size_t bytes = sizeof(kq::c2_t<T>) * 3 * (size_t)g.Vh;
-        for (int k = 0; k < noShifts; ++k)
-            KQ_HIP_CHECK(hipMemcpyAsync(sol[k]->DevicePtr(), run.x_lo(0, k), bytes, hipMemcpyDeviceToDevice, st));
-        KQ_HIP_CHECK(hipStreamSynchronize(st));
+        auto solve = [&](auto lng) {
+            kq::CgmRunner<T, T, LK, decltype(lng)> run(g, parity, f, lng, 1, noShifts, st, tune);
+            run.enable_deferred(kq::deferred_slots<T>(1, g.Vh));
+            const std::vector<double> tolk(noShifts, tol);
+            run.add_system(noShifts, shifts, tolk.data(), maxIters, src.DevicePtr(), nullptr);
+            run.run();
+            const size_t bytes = sizeof(kq::c2_t<T>) * 3 * (size_t)g.Vh;
+            for (int k = 0; k < noShifts; ++k)
+                KQ_HIP_CHECK(
+                    hipMemcpyAsync(sol[k]->DevicePtr(), run.x_lo(0, k), bytes, hipMemcpyDeviceToDevice, st));
+            KQ_HIP_CHECK(hipStreamSynchronize(st));
+            return run.state(0).iter;
+        };
+        iter = h->KernelLinks(2) != nullptr ? solve(lr) : solve(l);
     } else {
         iter = cgm_host<T, VecT>(this->dirac, noShifts, shifts, src, sol, tol, maxIters);
     }
