@@ -28,6 +28,7 @@ SOURCES = [
     "kq_dslash_fp32.hip",
     "kq_dslash_mixed.hip",
     "kq_dslash_16.hip",
+    "kq_dslash_u3.hip",
     "kq_cgm_kernels.hip",
     "kq_layout.hip",
     "kq_ptnet.hip",

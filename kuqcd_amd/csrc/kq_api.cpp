@@ -102,6 +102,20 @@ LinkF link_f(const GaugeSlot &s)
     return LinkF{s.kf.as<char>(), nb64_of(Vh)};
 }
 
+// run a packer that counts the links without its form; true when none is missing it
+static bool all_links_pass(void (*pack)(const double2 *, char *, uint32_t, int *, hipStream_t), const double2 *canon,
+                           DevBuf &dst, uint32_t Vh)
+{
+    Context &c = ctx();
+    DevBuf fail(sizeof(int));
+    KQ_HIP_CHECK(hipMemsetAsync(fail.as<void>(), 0, sizeof(int), c.stream));
+    pack(canon, dst.as<char>(), Vh, fail.as<int>(), c.stream);
+    int nfail = 0;
+    KQ_HIP_CHECK(hipMemcpyAsync(&nfail, fail.as<void>(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+    return nfail == 0;
+}
+
 DerivedFormats &derived(int slot, bool want_f32, bool want_f16)
 {
     Context &c = ctx();
@@ -115,35 +129,36 @@ DerivedFormats &derived(int slot, bool want_f32, bool want_f16)
         d.lng_f.alloc(link_field_bytes<LinkF>(Vh));
         canon_to_linkF<double>(f.soa.as<double2>(), d.fat_f.as<char>(), Vh, c.stream);
         canon_to_linkF<double>(l.soa.as<double2>(), d.lng_f.as<char>(), Vh, c.stream);
-        // long links also as 12 reals when every one is b * SU(3)
-        DevBuf fail(sizeof(int));
-        KQ_HIP_CHECK(hipMemsetAsync(fail.as<void>(), 0, sizeof(int), c.stream));
+        // long links also as 12 reals when every one is b * SU(3), or w * SU(3) with w complex
         d.lng_f12.alloc(link_field_bytes<LinkF12>(Vh));
-        canon_to_linkF12(l.soa.as<double2>(), d.lng_f12.as<char>(), Vh, fail.as<int>(), c.stream);
-        int nfail = 0;
-        KQ_HIP_CHECK(hipMemcpyAsync(&nfail, fail.as<void>(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
-        d.f32_long12 = (nfail == 0);
-        if (!d.f32_long12)
-            d.lng_f12.release();
+        if (all_links_pass(canon_to_linkF12, l.soa.as<double2>(), d.lng_f12, Vh)) {
+            d.f_long = kLong12;
+        } else {
+            d.lng_f12.alloc(link_field_bytes<LinkF12C>(Vh));
+            if (all_links_pass(canon_to_linkF12C, l.soa.as<double2>(), d.lng_f12, Vh))
+                d.f_long = kLong12C;
+            else
+                d.lng_f12.release();
+        }
         d.f32 = true;
     }
     if (want_f16 && !d.f16) {
         d.fat_h.alloc(link_field_bytes<LinkH>(Vh));
         canon_to_linkH(f.soa.as<double2>(), d.fat_h.as<char>(), Vh, c.stream);
-        // long links: try 12-real storage, fall back to 18 reals
-        DevBuf fail(sizeof(int));
-        KQ_HIP_CHECK(hipMemsetAsync(fail.as<void>(), 0, sizeof(int), c.stream));
+        // long links: 12 reals with a real scale, else with a complex scale, else 18 reals
         d.lng_h.alloc(link_field_bytes<LinkH12>(Vh));
-        canon_to_linkH12(l.soa.as<double2>(), d.lng_h.as<char>(), Vh, fail.as<int>(), c.stream);
-        int nfail = 0;
-        KQ_HIP_CHECK(hipMemcpyAsync(&nfail, fail.as<void>(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
-        d.long12 = (nfail == 0);
-        if (!d.long12) {
-            d.lng_h.alloc(link_field_bytes<LinkH>(Vh));
-            canon_to_linkH(l.soa.as<double2>(), d.lng_h.as<char>(), Vh, c.stream);
-            KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+        if (all_links_pass(canon_to_linkH12, l.soa.as<double2>(), d.lng_h, Vh)) {
+            d.h_long = kLong12;
+        } else {
+            d.lng_h.alloc(link_field_bytes<LinkH12C>(Vh));
+            if (all_links_pass(canon_to_linkH12C, l.soa.as<double2>(), d.lng_h, Vh)) {
+                d.h_long = kLong12C;
+            } else {
+                d.lng_h.alloc(link_field_bytes<LinkH>(Vh));
+                canon_to_linkH(l.soa.as<double2>(), d.lng_h.as<char>(), Vh, c.stream);
+                KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
+                d.h_long = kLong18;
+            }
         }
         d.f16 = true;
     }

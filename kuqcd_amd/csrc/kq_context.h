@@ -31,19 +31,24 @@ struct GaugeSlot {
 };
 
 // Lower-precision kernel formats of a double gauge pair, made on first use.
+// Long links in 12 reals: a real multiple of an SU(3) matrix (kLong12), a complex one (kLong12C,
+// U(3) products), else 18 reals (kLong18)
+enum LongForm { kLong18 = 0, kLong12 = 1, kLong12C = 2 };
 struct DerivedFormats {
     bool f32 = false, f16 = false;
-    bool long12 = false;      // FP16 long links stored as 12 reals (LinkH12), else LinkH
-    bool f32_long12 = false;  // lng_f12 holds the long links as LinkF12
-    DevBuf fat_f, lng_f;      // LinkF
-    DevBuf lng_f12;           // LinkF12 (when f32_long12)
-    DevBuf fat_h, lng_h;      // LinkH / LinkH12
+    int h_long = kLong18;  // 16-bit long links in lng_h: LinkH, LinkH12 or LinkH12C
+    int f_long = kLong18;  // FP32 long links: LinkF (lng_f), or LinkF12 / LinkF12C (lng_f12)
+    DevBuf fat_f, lng_f;   // LinkF
+    DevBuf lng_f12;        // LinkF12 or LinkF12C
+    DevBuf fat_h, lng_h;   // LinkH / LinkH12 / LinkH12C
 };
 
 inline LinkF as_linkF(const DevBuf &b, uint32_t Vh) { return LinkF{b.as<char>(), nb64_of(Vh)}; }
 inline LinkH as_linkH(const DevBuf &b, uint32_t Vh) { return LinkH{b.as<char>(), nb64_of(Vh)}; }
 inline LinkH12 as_linkH12(const DevBuf &b, uint32_t Vh) { return LinkH12{b.as<char>(), nb64_of(Vh)}; }
 inline LinkF12 as_linkF12(const DevBuf &b, uint32_t Vh) { return LinkF12{b.as<char>(), nb64_of(Vh)}; }
+inline LinkF12C as_linkF12C(const DevBuf &b, uint32_t Vh) { return LinkF12C{b.as<char>(), nb64_of(Vh)}; }
+inline LinkH12C as_linkH12C(const DevBuf &b, uint32_t Vh) { return LinkH12C{b.as<char>(), nb64_of(Vh)}; }
 
 struct Context {
     bool device_ready = false;
@@ -84,42 +89,47 @@ LinkF link_f(const GaugeSlot &s);
 DerivedFormats &derived(int slot, bool want_f32, bool want_f16);
 
 // fn(fat, lng) with the device link formats of a mixed mode of a double slot:
-// KQ_MODE_MIXED_FP32 -> LinkF fat, LinkF12 long links if they are b * SU(3) else LinkF;
-// KQ_MODE_MIXED_FP16 -> LinkH fat, LinkH12 long links if they are b * SU(3) else LinkH;
+// KQ_MODE_MIXED_FP32 -> LinkF fat; long links LinkF12 (b * SU(3)), LinkF12C (w * SU(3), w complex)
+//                       or LinkF;
+// KQ_MODE_MIXED_FP16 -> LinkH fat; long links LinkH12, LinkH12C or LinkH;
 // KQ_MODE_MIXED_FP32_L16 -> LinkF fat, long links as in KQ_MODE_MIXED_FP16.
-// Returns whether the long links use 12 reals.
 inline bool is_mixed_mode(int mode)
 {
     return mode == KQ_MODE_MIXED_FP32 || mode == KQ_MODE_MIXED_FP16 || mode == KQ_MODE_MIXED_FP32_L16;
 }
 
-template <class Fn> bool with_mixed_links(int slot, int mode, Fn &&fn)
+template <class FAT, class Fn> void with_h_long(const DerivedFormats &d, FAT fat, uint32_t Vh, Fn &&fn)
+{
+    if (d.h_long == kLong12)
+        fn(fat, as_linkH12(d.lng_h, Vh));
+    else if (d.h_long == kLong12C)
+        fn(fat, as_linkH12C(d.lng_h, Vh));
+    else
+        fn(fat, as_linkH(d.lng_h, Vh));
+}
+
+template <class Fn> void with_mixed_links(int slot, int mode, Fn &&fn)
 {
     const uint32_t Vh = slot_geom(slot).Vh;
     if (mode == KQ_MODE_MIXED_FP32_L16) {
         DerivedFormats &d = derived(slot, true, true);
-        if (d.long12)
-            fn(as_linkF(d.fat_f, Vh), as_linkH12(d.lng_h, Vh));
-        else
-            fn(as_linkF(d.fat_f, Vh), as_linkH(d.lng_h, Vh));
-        return d.long12;
+        with_h_long(d, as_linkF(d.fat_f, Vh), Vh, fn);
+        return;
     }
     if (mode == KQ_MODE_MIXED_FP16) {
         DerivedFormats &d = derived(slot, false, true);
-        if (d.long12)
-            fn(as_linkH(d.fat_h, Vh), as_linkH12(d.lng_h, Vh));
-        else
-            fn(as_linkH(d.fat_h, Vh), as_linkH(d.lng_h, Vh));
-        return d.long12;
+        with_h_long(d, as_linkH(d.fat_h, Vh), Vh, fn);
+        return;
     }
     if (mode != KQ_MODE_MIXED_FP32)
         throw std::invalid_argument("link format must be a mixed mode (KQ_MODE_MIXED_*)");
     DerivedFormats &d = derived(slot, true, false);
-    if (d.f32_long12)
+    if (d.f_long == kLong12)
         fn(as_linkF(d.fat_f, Vh), as_linkF12(d.lng_f12, Vh));
+    else if (d.f_long == kLong12C)
+        fn(as_linkF(d.fat_f, Vh), as_linkF12C(d.lng_f12, Vh));
     else
         fn(as_linkF(d.fat_f, Vh), as_linkF(d.lng_f, Vh));
-    return d.f32_long12;
 }
 
 template <class Fn> inline void with_fp64_links(int slot, Fn &&fn)

@@ -255,6 +255,80 @@ struct LinkF12 {
     }
 };
 
+// Row 2 of an SU(3) matrix from rows 0 and 1 (v[0..2], v[3..5]): conj(row0 x row1), times the
+// complex scale w: u[6 + c] = w conj(row0 x row1)_c
+template <typename R>
+__device__ inline void su3_row2_scaled(const c2_t<R> v[6], R wx, R wy, c2_t<R> u[9])
+{
+    using V2 = c2_t<R>;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        const V2 a1 = v[c1], b2 = v[3 + c2], a2 = v[c2], b1 = v[3 + c1];
+        const R re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
+        const R im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
+        u[6 + c] = mk<V2>(wx * re + wy * im, wy * re - wx * im);
+    }
+}
+
+// FP32 long links that are complex (not real) multiples of SU(3) matrices: a HISQ long link is
+// c_Naik times a product of three U(3) links, whose determinant carries a phase. Rows 0 and 1 of
+// V in SU(3) and w with w^3 = det U, U = w V. Chunks: 3 x 16 B (complex 0..5), 8 B (w): 56 B.
+struct LinkF12C {
+    using R = float;
+    static constexpr int kBytes = 56;
+    static constexpr size_t kBlock = 64 * kBytes;
+    const char *p;
+    uint32_t nb64;
+    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
+    {
+        const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
+        const uint32_t l = s & 63;
+        float2 v[6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const kq_v4f w4 = ld_nt(reinterpret_cast<const kq_v4f *>(b + 1024 * c) + l);
+            v[2 * c] = mk<float2>(w4.x, w4.y);
+            v[2 * c + 1] = mk<float2>(w4.z, w4.w);
+        }
+        const kq_v2f w = ld_nt(reinterpret_cast<const kq_v2f *>(b + 3072) + l);
+        su3_row2_scaled<float>(v, w.x, w.y, u);
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            u[k] = mk<float2>(w.x * v[k].x - w.y * v[k].y, w.x * v[k].y + w.y * v[k].x);
+    }
+};
+
+// The same in 16 bits: rows 0 and 1 of V as fixed point q / 32767 and w in FP32.
+// Chunks: 16 B (complex 0..3), 8 B (complex 4, 5), 8 B (w): 32 B.
+struct LinkH12C {
+    using R = float;
+    static constexpr int kBytes = 32;
+    static constexpr size_t kBlock = 64 * kBytes;
+    const char *p;
+    uint32_t nb64;
+    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
+    {
+        const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
+        const uint32_t l = s & 63;
+        const kq_v4u c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
+        const kq_v2u c1 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 1024) + l);
+        const kq_v2f w = ld_nt(reinterpret_cast<const kq_v2f *>(b + 1536) + l);
+        constexpr float kQ = 1.0f / 32767.0f;
+        const unsigned int h[6] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y};
+        float2 v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const float2 f = q2f(h[k]);
+            v[k] = mk<float2>(f.x * kQ, f.y * kQ);
+        }
+        su3_row2_scaled<float>(v, w.x, w.y, u);
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            u[k] = mk<float2>(w.x * v[k].x - w.y * v[k].y, w.x * v[k].y + w.y * v[k].x);
+    }
+};
+
 inline uint32_t nb64_of(uint32_t Vh) { return (Vh + 63) / 64; }
 // bytes of one link field (both parities, 4 directions) in format LK
 template <class LK> inline size_t link_field_bytes(uint32_t Vh) { return (size_t)8 * nb64_of(Vh) * LK::kBlock; }

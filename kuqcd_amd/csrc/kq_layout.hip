@@ -197,6 +197,95 @@ __global__ void k_canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh,
     reinterpret_cast<float *>(b + 1536)[l] = (float)bsc;
 }
 
+// U = w V with w = principal cube root of det U, V = U / w; true when the rebuilt w (V0, V1,
+// conj(V0 x V1)) matches U to tol * max|U_ij| (U a complex multiple of an SU(3) matrix)
+__device__ inline bool su3_factor_c(const double2 *canon, uint32_t qm, uint32_t s, uint32_t Vh, double2 v[6],
+                                    double2 &w, double tol)
+{
+    double2 m[9];
+    double amax = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        m[k] = canon[((size_t)qm * 9 + k) * Vh + s];
+        amax = fmax(amax, fmax(fabs(m[k].x), fabs(m[k].y)));
+    }
+    const double2 d = det3(m);
+    const double r = cbrt(hypot(d.x, d.y)), ph = atan2(d.y, d.x) / 3.0;
+    w = mk<double2>(r * cos(ph), r * sin(ph));
+    const double inv = r > 0.0 ? 1.0 / (r * r) : 0.0;
+    for (int k = 0; k < 6; ++k)
+        v[k] = mk<double2>((m[k].x * w.x + m[k].y * w.y) * inv, (m[k].y * w.x - m[k].x * w.y) * inv);
+    double err = 0.0;
+    for (int k = 0; k < 6; ++k) {
+        const double2 u = mk<double2>(w.x * v[k].x - w.y * v[k].y, w.x * v[k].y + w.y * v[k].x);
+        err = fmax(err, fmax(fabs(u.x - m[k].x), fabs(u.y - m[k].y)));
+    }
+    for (int c = 0; c < 3; ++c) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        const double2 a1 = v[c1], b2 = v[3 + c2], a2 = v[c2], b1 = v[3 + c1];
+        const double re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
+        const double im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
+        const double2 u = mk<double2>(w.x * re + w.y * im, w.y * re - w.x * im);
+        err = fmax(err, fmax(fabs(u.x - m[6 + c].x), fabs(u.y - m[6 + c].y)));
+    }
+    return r > 0.0 && err <= tol * amax;
+}
+
+__global__ void k_canon_to_linkF12C(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64, int *fail)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)8 * Vh)
+        return;
+    const uint32_t qm = (uint32_t)(t / Vh), s = (uint32_t)(t - (size_t)qm * Vh);
+    double2 v[6], w;
+    if (!su3_factor_c(canon, qm, s, Vh, v, w, 1e-6))
+        atomicAdd(fail, 1);
+    char *b = dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkF12C::kBlock);
+    const uint32_t l = s & 63;
+    for (int c = 0; c < 3; ++c)
+        reinterpret_cast<float4 *>(b + 1024 * c)[l] =
+            make_float4((float)v[2 * c].x, (float)v[2 * c].y, (float)v[2 * c + 1].x, (float)v[2 * c + 1].y);
+    reinterpret_cast<float2 *>(b + 3072)[l] = make_float2((float)w.x, (float)w.y);
+}
+
+__global__ void k_canon_to_linkH12C(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64, int *fail)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)8 * Vh)
+        return;
+    const uint32_t qm = (uint32_t)(t / Vh), s = (uint32_t)(t - (size_t)qm * Vh);
+    double2 v[6], w;
+    if (!su3_factor_c(canon, qm, s, Vh, v, w, 1e-6))
+        atomicAdd(fail, 1);
+    unsigned int h[6];
+    for (int k = 0; k < 6; ++k)
+        h[k] = pack_q2(v[k].x, v[k].y);
+    char *b = dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkH12C::kBlock);
+    const uint32_t l = s & 63;
+    reinterpret_cast<uint4 *>(b)[l] = make_uint4(h[0], h[1], h[2], h[3]);
+    reinterpret_cast<uint2 *>(b + 1024)[l] = make_uint2(h[4], h[5]);
+    reinterpret_cast<float2 *>(b + 1536)[l] = make_float2((float)w.x, (float)w.y);
+}
+
+void canon_to_linkF12C(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+{
+    const size_t n = (size_t)8 * Vh;
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_canon_to_linkF12C, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
+                       nb64_of(Vh), fail);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+void canon_to_linkH12C(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+{
+    const size_t n = (size_t)8 * Vh;
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_canon_to_linkH12C, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
+                       nb64_of(Vh), fail);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
 // LinkD14: w = principal cube root of det U (so V = U / w has det 1), rows 0 and 1 of V and w.
 // A link counts in *fail when the reconstruction w (V0, V1, conj(V0 x V1)), formed as
 // LinkD14::load forms it, differs from U by more than 1e-13 max|U_ij| in any entry: U is then not

@@ -158,54 +158,43 @@ static double time_dslash(LF fat, LL lng, const Geom &g, int nrhs, int nr, int n
     return (double)ms / iters;
 }
 
-// Algorithmic bytes per output site: every link of the lattice once
-// (8 fat + 8 long per site), one source and one result vector per right-hand side.
-// long12: long links in 12 reals + scale (FP32 / 16-bit formats) or 14 reals (FP64, LinkD14).
-static double site_bytes(int mode, bool dbl_gauge, bool long12, int nrhs)
+// Algorithmic bytes per output site of a hopping term with link formats LF, LL and vectors of
+// real type R: every link of the lattice once (8 fat + 8 long per site), one source and one
+// result vector per right-hand side
+template <class LF, class LL, typename R> static double site_bytes(int nrhs)
 {
-    double bl_f, bl_l, bv;
-    if (mode == KQ_MODE_MIXED_FP16 || mode == KQ_MODE_MIXED_FP32_L16) {
-        bl_f = mode == KQ_MODE_MIXED_FP16 ? 36 + 4 : 72;
-        bl_l = long12 ? 24 + 4 : 36 + 4;
-        bv = 24;
-    } else if (mode == KQ_MODE_MIXED_FP32 || !dbl_gauge) {
-        bl_f = 72;
-        bl_l = long12 ? 48 + 4 : 72;  // LinkF12 (mixed FP32 mode, or a float pair's long links)
-        bv = 24;
-    } else {
-        bl_f = 144;
-        bl_l = long12 ? 112 : 144;
-        bv = 48;
-    }
-    return 8 * bl_f + 8 * bl_l + nrhs * 2 * bv;
+    return 8.0 * (LF::kBytes + LL::kBytes) + nrhs * 2.0 * 3 * sizeof(c2_t<R>);
 }
 
-// dispatch a timing run of the format `mode` on slot
-static double time_mode(int slot, int mode, int nrhs, int nr, int nt, int iters, bool *long12, int cap = 0)
+// dispatch a timing run of the format `mode` on slot; *bsite: its algorithmic bytes per site
+static double time_mode(int slot, int mode, int nrhs, int nr, int nt, int iters, double *bsite, int cap = 0)
 {
     Context &c = ctx();
     const Geom g = slot_geom(slot);
     const bool dbl = c.fat[slot].prec == KUQCD_DOUBLE;
-    *long12 = false;
+    double ms = 0.0;
     if (mode == KQ_MODE_REFERENCE || mode == KQ_MODE_FP64) {
-        if (dbl) {
-            double ms = 0.0;
-            *long12 = c.lng[slot].lrecon;
+        if (dbl)
             with_fp64_links(slot, [&](auto fat, auto lng) {
-                ms = time_dslash<decltype(fat), decltype(lng), double>(fat, lng, g, nrhs, nr, nt, iters, cap);
+                using LF = decltype(fat);
+                using LL = decltype(lng);
+                ms = time_dslash<LF, LL, double>(fat, lng, g, nrhs, nr, nt, iters, cap);
+                *bsite = site_bytes<LF, LL, double>(nrhs);
             });
-            return ms;
-        }
-        double ms = 0.0;
-        *long12 = c.lng[slot].lrecon;
-        with_fp32_links(slot, [&](auto fat, auto lng) {
-            ms = time_dslash<decltype(fat), decltype(lng), float>(fat, lng, g, nrhs, nr, nt, iters, cap);
-        });
+        else
+            with_fp32_links(slot, [&](auto fat, auto lng) {
+                using LF = decltype(fat);
+                using LL = decltype(lng);
+                ms = time_dslash<LF, LL, float>(fat, lng, g, nrhs, nr, nt, iters, cap);
+                *bsite = site_bytes<LF, LL, float>(nrhs);
+            });
         return ms;
     }
-    double ms = 0.0;
-    *long12 = with_mixed_links(slot, mode, [&](auto fat, auto lng) {
-        ms = time_dslash<decltype(fat), decltype(lng), float>(fat, lng, g, nrhs, nr, nt, iters, cap);
+    with_mixed_links(slot, mode, [&](auto fat, auto lng) {
+        using LF = decltype(fat);
+        using LL = decltype(lng);
+        ms = time_dslash<LF, LL, float>(fat, lng, g, nrhs, nr, nt, iters, cap);
+        *bsite = site_bytes<LF, LL, float>(nrhs);
     });
     return ms;
 }
@@ -233,7 +222,7 @@ std::pair<int, int> tuned_launch(int slot, int mode, int nrhs)
         tg.fill(-1.0);
     double best = 1e30;
     std::pair<int, int> pick{1, 256};
-    bool l12 = false;
+    double bsite = 0.0;
     for (int i = 0; i < 5; ++i) {
         const int nr = kNr[i];
         if (nr > 1 && (nr >= 2 * nrhs || (mode == KQ_MODE_FP64 && nr > 2)))
@@ -243,7 +232,7 @@ std::pair<int, int> tuned_launch(int slot, int mode, int nrhs)
             if (ms < 0) {
                 int best_cap = 0;
                 for (int cap : kCap) {
-                    const double t = time_mode(slot, mode, nr, nr, kNt[j], 8, &l12, cap);
+                    const double t = time_mode(slot, mode, nr, nr, kNt[j], 8, &bsite, cap);
                     if (ms < 0 || t < ms) {
                         ms = t;
                         best_cap = cap;
@@ -348,12 +337,11 @@ KuQCDStatus kq_bench_dslash(int mode, int nrhs, int nr, int nt, int iters, KqBen
             nt = nt > 0 ? nt : p.second;
             cap = tuned_cap(slot, mode, nr, nt);
         }
-        bool l12 = false;
-        const double ms = time_mode(slot, mode, nrhs, nr, nt, iters, &l12, cap);
+        double bsite = 0.0;
+        const double ms = time_mode(slot, mode, nrhs, nr, nt, iters, &bsite, cap);
         const Geom g = slot_geom(slot);
-        const bool dbl = ctx().fat[slot].prec == KUQCD_DOUBLE;
         out->ms = ms;
-        out->bytes = site_bytes(mode, dbl, l12, nrhs) * (double)g.Vh;
+        out->bytes = bsite * (double)g.Vh;
         out->gbs = out->bytes / (ms * 1e-3) / 1e9;
         out->gflops = 1146.0 * (double)g.Vh * nrhs / (ms * 1e-3) / 1e9;
         out->hbm_frac = out->gbs / 8000.0;

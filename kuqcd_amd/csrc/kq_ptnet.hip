@@ -717,5 +717,8 @@ KQ_PT_INST(LinkH, LinkH12)
 KQ_PT_INST(LinkH, LinkH)
 KQ_PT_INST(LinkF, LinkH12)
 KQ_PT_INST(LinkF, LinkH)
+KQ_PT_INST(LinkF, LinkF12C)
+KQ_PT_INST(LinkH, LinkH12C)
+KQ_PT_INST(LinkF, LinkH12C)
 
 }  // namespace kq

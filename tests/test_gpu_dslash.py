@@ -172,3 +172,57 @@ def test_dslash_fp32_long_link_formats(gpu, variant, b_site):
         assert kq.dslash(src, dst, parity) == 0, kq.last_error()
         assert rel_err(dst, oracle.dslash(fat32, lng32, src, lat, parity)) <= 1e-5, (variant, parity)
     kq.kuqcd_gauge_free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,tol,b_su3,b_u3", [(2, 1e-5, 1040, 1072), (3, 3e-3, 592, 624), (4, 5e-4, 848, 880)])
+def test_dslash_mixed_long_link_forms(gpu, mode, tol, b_su3, b_u3):
+    """The mixed modes keep long links in 12 reals with a real scale when they are real multiples
+    of SU(3) matrices, and with a complex scale when they are complex multiples (U(3) products, as
+    HISQ's reunitarised links give: a phase per link here): 8 x 52 / 28 bytes per long link, or
+    8 x 56 / 32. Both agree with the FP64 oracle to the mode's tolerance."""
+    kq = gpu
+    lat = (8, 8, 8, 8)
+    fat, lng = kq.gen_gauge(lat, kind=kq.KQ_GEN_HAAR, seed_fat=41, seed_long=42)
+    rng = np.random.default_rng(43)
+    ph = np.exp(1j * rng.uniform(0, 2 * np.pi, lng.shape[:2]))[..., None]
+    z = (lng[..., 0] + 1j * lng[..., 1]) * ph
+    lng_u3 = np.ascontiguousarray(np.stack([z.real, z.imag], axis=-1))
+    for links, b_site in ((lng, b_su3), (lng_u3, b_u3)):
+        kq.load_gauge_pair(fat, links, lat)
+        st, r = kq.bench_dslash(mode, 1, 1, 256, 1)
+        assert st == 0, kq.last_error()
+        assert round(r.bytes / (np.prod(lat) // 2)) == b_site, (mode, b_site)
+        src = kq.gen_source(lat, 0, seed=44)
+        dst = np.empty_like(src)
+        assert kq.dslash(src, dst, 1, mode) == 0, kq.last_error()
+        assert rel_err(dst, oracle.dslash(fat, links, src, lat, 1)) <= tol, (mode, b_site)
+        kq.kuqcd_gauge_free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [2, 4])
+def test_mixed_solve_u3_long_links(gpu, mode):
+    """A mixed solve whose long links are U(3) products (12 reals with a complex scale) reaches
+    every shift's FP64 target and matches the FP64 solve of the same links."""
+    kq = gpu
+    lat = (8, 8, 8, 8)
+    fat, lng = kq.gen_gauge(lat, kind=kq.KQ_GEN_HAAR, seed_fat=51, seed_long=52)
+    rng = np.random.default_rng(53)
+    ph = np.exp(1j * rng.uniform(0, 2 * np.pi, lng.shape[:2]))[..., None]
+    z = (lng[..., 0] + 1j * lng[..., 1]) * ph
+    lng = np.ascontiguousarray(np.stack([z.real, z.imag], axis=-1))
+    kq.load_gauge_pair(fat, lng, lat)
+    b = kq.gen_source(lat, 0, seed=54)
+    shifts = np.array([0.0025, 0.01, 0.1, 1.0])
+    out = {}
+    for m in (kq.KQ_MODE_FP64, mode):
+        x = [np.zeros_like(b) for _ in shifts]
+        prm = kq.InvertParam(shifts=shifts, tol=0.0, max_iter=5000)
+        st, rep = kq.solve_ex(x, b, prm, kq.default_options(mode=m, refine=1, rel_tol=1e-10))
+        assert st == kq.KUQCD_SUCCESS, kq.last_error()
+        assert max(rep.true_rel_res[:len(shifts)]) <= 1e-10, m
+        out[m] = x
+    for k in range(len(shifts)):
+        assert rel_err(out[mode][k], out[kq.KQ_MODE_FP64][k]) <= 1e-7, k
+    kq.kuqcd_gauge_free()
