@@ -90,11 +90,7 @@ LinkD14 link_d14(const GaugeSlot &s)
     const uint32_t Vh = (uint32_t)((uint64_t)s.lat[0] * s.lat[1] * s.lat[2] * s.lat[3] / 2);
     return LinkD14{s.kfr.as<char>(), nb64_of(Vh)};
 }
-LinkF12 link_f12(const GaugeSlot &s)
-{
-    const uint32_t Vh = (uint32_t)((uint64_t)s.lat[0] * s.lat[1] * s.lat[2] * s.lat[3] / 2);
-    return LinkF12{s.kfr.as<char>(), nb64_of(Vh)};
-}
+
 
 LinkF link_f(const GaugeSlot &s)
 {
@@ -102,9 +98,11 @@ LinkF link_f(const GaugeSlot &s)
     return LinkF{s.kf.as<char>(), nb64_of(Vh)};
 }
 
+
 // run a packer that counts the links without its form; true when none is missing it
-static bool all_links_pass(void (*pack)(const double2 *, char *, uint32_t, int *, hipStream_t), const double2 *canon,
-                           DevBuf &dst, uint32_t Vh)
+template <typename C2>
+static bool all_links_pass(void (*pack)(const C2 *, char *, uint32_t, int *, hipStream_t), const C2 *canon, DevBuf &dst,
+                           uint32_t Vh)
 {
     Context &c = ctx();
     DevBuf fail(sizeof(int));
@@ -229,6 +227,7 @@ static KuQCDStatus init_from_stdin()
 }
 
 // ---------------------------------------------------------------- load
+
 template <typename HR>
 static void upload_links(const HR *h, GaugeSlot &slot, double scale, int prec, bool long_links)
 {
@@ -259,22 +258,25 @@ static void upload_links(const HR *h, GaugeSlot &slot, double scale, int prec, b
         canon_to_linkF<float>(slot.soa.as<float2>(), slot.kf.as<char>(), Vh, c.stream);
     }
     // long links also in fewer reals when every one has that form: 14 in double (a complex
-    // multiple of an SU(3) matrix, to 1e-13), 12 in float (a real multiple, to 1e-6)
+    // multiple of an SU(3) matrix, to 1e-13); 12 in float, with a real or else a complex scale
+    // (to 1e-6)
     if (long_links) {
-        DevBuf fail(sizeof(int));
-        KQ_HIP_CHECK(hipMemsetAsync(fail.as<void>(), 0, sizeof(int), c.stream));
+        slot.lform = kLong18;
         if (prec == KUQCD_DOUBLE) {
             slot.kfr.alloc(link_field_bytes<LinkD14>(Vh));
-            canon_to_linkD14(slot.soa.as<double2>(), slot.kfr.as<char>(), Vh, fail.as<int>(), c.stream);
+            if (all_links_pass(canon_to_linkD14, slot.soa.as<double2>(), slot.kfr, Vh))
+                slot.lform = kLong12C;
         } else {
             slot.kfr.alloc(link_field_bytes<LinkF12>(Vh));
-            canon_to_linkF12(slot.soa.as<float2>(), slot.kfr.as<char>(), Vh, fail.as<int>(), c.stream);
+            if (all_links_pass(canon_to_linkF12, slot.soa.as<float2>(), slot.kfr, Vh)) {
+                slot.lform = kLong12;
+            } else {
+                slot.kfr.alloc(link_field_bytes<LinkF12C>(Vh));
+                if (all_links_pass(canon_to_linkF12C, slot.soa.as<float2>(), slot.kfr, Vh))
+                    slot.lform = kLong12C;
+            }
         }
-        int nfail = 0;
-        KQ_HIP_CHECK(hipMemcpyAsync(&nfail, fail.as<void>(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
-        slot.lrecon = (nfail == 0);
-        if (!slot.lrecon)
+        if (slot.lform == kLong18)
             slot.kfr.release();
     }
     KQ_HIP_CHECK(hipStreamSynchronize(c.stream));
@@ -352,7 +354,7 @@ static KuQCDStatus load_gauge(void *h_gauge, KuQCDGaugeParam *param)
     gs.soa.release();
     gs.kf.release();
     gs.kfr.release();
-    gs.lrecon = false;
+    gs.lform = kLong18;
     free_derived(slot);
     for (int i = 0; i < 4; ++i)
         gs.lat[i] = param->lat[i];

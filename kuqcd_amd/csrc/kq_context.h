@@ -18,22 +18,24 @@ namespace kq {
 // pre-scaled (1/2 fat, -1/48 long): the canonical SoA copy
 // canon[((q*4 + mu)*9 + k)*Vh + s] from which every kernel format is made,
 // and the kernel format of the loaded precision (LinkD or LinkF).
+// Long links in fewer reals: a real multiple of an SU(3) matrix (kLong12), a complex one
+// (kLong12C, U(3) products), else 18 reals (kLong18)
+enum LongForm { kLong18 = 0, kLong12 = 1, kLong12C = 2 };
+
 struct GaugeSlot {
     bool loaded = false;
     int prec = 0;  // 4 or 8
     unsigned lat[4] = {0, 0, 0, 0};
     DevBuf soa;  // canonical, double2 or float2
     DevBuf kf;   // LinkD (double) or LinkF (float) format
-    // long links also held in fewer reals in kfr: LinkD14 (double), LinkF12 (float), when every
-    // link has that form
-    bool lrecon = false;
+    // long links also held in fewer reals in kfr when every link has that form: kLong12C in
+    // double (LinkD14, a complex multiple of an SU(3) matrix); kLong12 (LinkF12, a real multiple)
+    // or kLong12C (LinkF12C) in float
+    int lform = kLong18;
     DevBuf kfr;
 };
 
 // Lower-precision kernel formats of a double gauge pair, made on first use.
-// Long links in 12 reals: a real multiple of an SU(3) matrix (kLong12), a complex one (kLong12C,
-// U(3) products), else 18 reals (kLong18)
-enum LongForm { kLong18 = 0, kLong12 = 1, kLong12C = 2 };
 struct DerivedFormats {
     bool f32 = false, f16 = false;
     int h_long = kLong18;  // 16-bit long links in lng_h: LinkH, LinkH12 or LinkH12C
@@ -79,9 +81,8 @@ Geom slot_geom(int slot);
 
 LinkD link_d(const GaugeSlot &s);
 LinkD14 link_d14(const GaugeSlot &s);
-LinkF12 link_f12(const GaugeSlot &s);
 // fn(LinkD fat, lng) with the long links of the slot's double pair as LinkD14 when every one
-// has that form (GaugeSlot::lrecon), else as LinkD: the FP64 hopping term of the solvers, the FP64
+// has that form (GaugeSlot::lform), else as LinkD: the FP64 hopping term of the solvers, the FP64
 // verification and the operator entry points
 template <class Fn> inline void with_fp64_links(int slot, Fn &&fn);
 LinkF link_f(const GaugeSlot &s);
@@ -135,21 +136,26 @@ template <class Fn> void with_mixed_links(int slot, int mode, Fn &&fn)
 template <class Fn> inline void with_fp64_links(int slot, Fn &&fn)
 {
     Context &c = ctx();
-    if (c.lng[slot].lrecon)
+    if (c.lng[slot].lform != kLong18)
         fn(link_d(c.fat[slot]), link_d14(c.lng[slot]));
     else
         fn(link_d(c.fat[slot]), link_d(c.lng[slot]));
 }
 
-// fn(LinkF fat, lng) for a float pair: long links as LinkF12 when every one is a real multiple of
-// an SU(3) matrix (GaugeSlot::lrecon), else as LinkF (the float reference method and operators)
+// fn(LinkF fat, lng) for a float pair: long links as LinkF12 or LinkF12C when every one is a real
+// or a complex multiple of an SU(3) matrix (GaugeSlot::lform), else as LinkF (the float reference
+// method and operators)
 template <class Fn> inline void with_fp32_links(int slot, Fn &&fn)
 {
     Context &c = ctx();
-    if (c.lng[slot].lrecon)
-        fn(link_f(c.fat[slot]), link_f12(c.lng[slot]));
+    const GaugeSlot &l = c.lng[slot];
+    const uint32_t Vh = (uint32_t)((uint64_t)l.lat[0] * l.lat[1] * l.lat[2] * l.lat[3] / 2);
+    if (l.lform == kLong12)
+        fn(link_f(c.fat[slot]), as_linkF12(l.kfr, Vh));
+    else if (l.lform == kLong12C)
+        fn(link_f(c.fat[slot]), as_linkF12C(l.kfr, Vh));
     else
-        fn(link_f(c.fat[slot]), link_f(c.lng[slot]));
+        fn(link_f(c.fat[slot]), link_f(l));
 }
 
 // device address of the no-allocation entry point's static scratch array on the current

@@ -36,6 +36,7 @@ void canon_to_linkF12(const float2 *canon, char *dst, uint32_t Vh, int *fail, hi
 // long links as complex multiples of SU(3) matrices (LinkF12C, LinkH12C); *fail counts the links
 // without that form to 1e-6
 void canon_to_linkF12C(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);
+void canon_to_linkF12C(const float2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);
 void canon_to_linkH12C(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st);
 
 // host-layout colour vector (device copy) <-> SoA vector of right-hand side r

@@ -199,13 +199,15 @@ __global__ void k_canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh,
 
 // U = w V with w = principal cube root of det U, V = U / w; true when the rebuilt w (V0, V1,
 // conj(V0 x V1)) matches U to tol * max|U_ij| (U a complex multiple of an SU(3) matrix)
-__device__ inline bool su3_factor_c(const double2 *canon, uint32_t qm, uint32_t s, uint32_t Vh, double2 v[6],
-                                    double2 &w, double tol)
+template <typename C2>
+__device__ inline bool su3_factor_c(const C2 *canon, uint32_t qm, uint32_t s, uint32_t Vh, double2 v[6], double2 &w,
+                                    double tol)
 {
     double2 m[9];
     double amax = 0.0;
     for (int k = 0; k < 9; ++k) {
-        m[k] = canon[((size_t)qm * 9 + k) * Vh + s];
+        const C2 c = canon[((size_t)qm * 9 + k) * Vh + s];
+        m[k] = mk<double2>((double)c.x, (double)c.y);
         amax = fmax(amax, fmax(fabs(m[k].x), fabs(m[k].y)));
     }
     const double2 d = det3(m);
@@ -230,7 +232,8 @@ __device__ inline bool su3_factor_c(const double2 *canon, uint32_t qm, uint32_t 
     return r > 0.0 && err <= tol * amax;
 }
 
-__global__ void k_canon_to_linkF12C(const double2 *canon, char *dst, uint32_t Vh, uint32_t nb64, int *fail)
+template <typename C2>
+__global__ void k_canon_to_linkF12C(const C2 *canon, char *dst, uint32_t Vh, uint32_t nb64, int *fail)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)8 * Vh)
@@ -266,14 +269,25 @@ __global__ void k_canon_to_linkH12C(const double2 *canon, char *dst, uint32_t Vh
     reinterpret_cast<float2 *>(b + 1536)[l] = make_float2((float)w.x, (float)w.y);
 }
 
-void canon_to_linkF12C(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+template <typename C2> static void canon_to_linkF12C_t(const C2 *canon, char *dst, uint32_t Vh, int *fail,
+                                                       hipStream_t st)
 {
     const size_t n = (size_t)8 * Vh;
     if (n == 0)
         return;
-    hipLaunchKernelGGL(k_canon_to_linkF12C, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
+    hipLaunchKernelGGL(k_canon_to_linkF12C<C2>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, canon, dst, Vh,
                        nb64_of(Vh), fail);
     KQ_HIP_CHECK(hipGetLastError());
+}
+
+void canon_to_linkF12C(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+{
+    canon_to_linkF12C_t(canon, dst, Vh, fail, st);
+}
+
+void canon_to_linkF12C(const float2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)
+{
+    canon_to_linkF12C_t(canon, dst, Vh, fail, st);
 }
 
 void canon_to_linkH12C(const double2 *canon, char *dst, uint32_t Vh, int *fail, hipStream_t st)

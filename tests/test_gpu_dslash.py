@@ -145,12 +145,13 @@ def test_dslash_fp64_long_link_formats(gpu, variant, b_site):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant,b_site", [("su3", 1040), ("u3", 1200), ("perturbed", 1200)])
+@pytest.mark.parametrize("variant,b_site", [("su3", 1040), ("u3", 1072), ("perturbed", 1200)])
 def test_dslash_fp32_long_link_formats(gpu, variant, b_site):
-    """A float pair's long links are kept in 12 reals (two rows of an SU(3) matrix and a real
-    scale) when every link is a real multiple of an SU(3) matrix to 1e-6; U(3) links with a phase
-    in their determinant and perturbed links keep 18 reals. The operator is the oracle's float
-    operator to 1e-5 either way (8 x 72 fat + 8 x 52 or 72 long + 2 x 24 bytes per site)."""
+    """A float pair's long links are kept in 12 reals (two rows of an SU(3) matrix) with a real
+    scale when every link is a real multiple of an SU(3) matrix to 1e-6, with a complex scale when
+    they are complex multiples (U(3) links with a phase in their determinant); perturbed links keep
+    18 reals. The operator is the oracle's float operator to 1e-5 each time (8 x 72 fat + 8 x 52,
+    56 or 72 long + 2 x 24 bytes per site)."""
     kq = gpu
     lat = (8, 8, 8, 8)
     fat, lng = kq.gen_gauge(lat, kind=kq.KQ_GEN_HAAR, seed_fat=31, seed_long=32)
