@@ -264,4 +264,17 @@ template <int NT> __device__ inline uint32_t tile_site(const Geom &g, uint32_t b
     return valid ? s0 : 0;
 }
 
+// x-rows of one (x, y) plane that each workgroup of nt threads covers in the tile order, when it
+// covers whole rows of a single plane: the tile order is active, an XCD's slab is the whole
+// plane (one y range), and nt is a multiple of the row length Lx/2 that divides the plane. The
+// hopping terms then stage the workgroup's rows and their +-3 y-neighbour rows of every source
+// in LDS (16-byte loads need an even row length). 0 when the workgroups do not have this shape.
+__host__ inline int stage_rows(const Geom &g, int nt)
+{
+    if (!g.tile || g.Vh % (8u * (uint32_t)nt) != 0 || g.lyp != 0 || g.lxh <= 0 || g.lxh % 2 != 0 ||
+        nt % g.lxh != 0 || g.plane % (uint32_t)nt != 0 || g.L[1] < 3)
+        return 0;
+    return nt / g.lxh;
+}
+
 }  // namespace kq

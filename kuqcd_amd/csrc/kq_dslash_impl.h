@@ -61,6 +61,40 @@ __device__ __forceinline__ void hop_bwd(const A &a, const LK &lk, uint32_t s, co
     }
 }
 
+// Row length Lx/2 of the lattices whose batched hopping terms stage rows in LDS (Lx = 32, as at
+// 32^4): a compile-time row length gives a static LDS size
+constexpr int kStageLxh = 16;
+
+// Hop along x (MU 0) or y (MU 1) of a workgroup whose source rows are staged in LDS:
+// st[j][c][k] holds colour c of right-hand side j at k = (y - y0 + 3) * Lx/2 + x/2 for the rows
+// y0 - 3 .. y0 + NT / (Lx/2) + 2 of the workgroup's plane (yl = y - y0). Same arithmetic, in the
+// same order, as hop_fwd / hop_bwd.
+template <class A, class LK, int NR, int MU, int H, bool BWD, int KR>
+__device__ __forceinline__ void hop_staged(const A &a, const LK &lk, uint32_t s, const SiteCoord &sc, int yl,
+                                           const typename A::V2 (*st)[3][KR], typename A::R acc[NR][6])
+{
+    using R = typename A::R;
+    constexpr int XH = kStageLxh;
+    const int h = BWD ? -H : H;
+    const int k = MU == 0 ? (yl + 3) * XH + (wrap_add(sc.c[0], h, a.g.L[0]) >> 1) : (yl + 3 + h) * XH + (sc.c[0] >> 1);
+    c2_t<R> U[9];
+    if (BWD)
+        lk.load(1 - a.dst_parity, MU, nb_index<MU>(a.g, s, sc, h), U);
+    else
+        lk.load(a.dst_parity, MU, s, U);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        typename A::V2 v[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            v[c] = st[j][c][k];
+        if (BWD)
+            udmv_sub(U, v, acc[j]);
+        else
+            umv_add(U, v, acc[j]);
+    }
+}
+
 // output store: streaming for the FP64 and FP32 link formats (measured 5% and 4%
 // faster at 32^4), plain for the 16-bit one (3% slower with streaming stores). Also
 // streaming where the next hopping term gathers the output (MdaggM's intermediate):
@@ -74,60 +108,15 @@ template <class LF, class V2> __device__ __forceinline__ void put(V2 *p, V2 v)
         st_nt(p, v);
 }
 
-template <class LF, class LL, typename SR, int NR, int EPI, int NT>
-__global__ void __launch_bounds__(NT) k_dslash(const DslashArgs<LF, LL, SR> a)
+// Epilogue of the hopping term: store D src (EPI_STORE), or dst = sigma aux - D src with the
+// per-workgroup partial sums of <aux, dst> (EPI_AXMY_DOT)
+template <class LF, typename SR, int NR, int EPI, int NT, class A, typename R>
+__device__ __forceinline__ void dslash_epilogue(const A &a, const R (&acc)[NR][6], c2_t<SR> *const (&dst)[NR],
+                                                const c2_t<SR> *const (&aux)[NR], const int (&rr)[NR], int j0,
+                                                int nact, uint32_t s, bool valid, double *lds)
 {
-    using A = DslashArgs<LF, LL, SR>;
-    using R = typename A::R;
     using V2 = c2_t<SR>;
-    __shared__ double lds[NT / kWave];
-    if (a.run != nullptr && *a.run == 0)
-        return;  // uniform over the grid
-    const int nact = a.nact ? *a.nact : a.nrhs;
-    const int j0 = blockIdx.y * NR;
-    if (j0 >= nact)
-        return;  // uniform over the block
-    int rr[NR];
-    // per right-hand-side base pointers, resolved once (uniform over the block)
-    const V2 *src[NR];
-    V2 *dst[NR];
-    const V2 *aux[NR];
-#pragma unroll
-    for (int j = 0; j < NR; ++j) {
-        const int jj = (j0 + j < nact) ? j0 + j : nact - 1;  // duplicate the last one as padding
-        rr[j] = a.rlist ? a.rlist[jj] : jj;
-        src[j] = a.src.p + a.src.base(rr[j]);
-        dst[j] = a.dst.p + a.dst.base(rr[j]);
-        aux[j] = EPI == EPI_AXMY_DOT ? a.aux.p + a.aux.base(rr[j]) : nullptr;
-    }
     const uint32_t Vh = a.g.Vh;
-    bool valid;
-    const uint32_t s = tile_site<NT>(a.g, blockIdx.x, gridDim.x, threadIdx.x, valid);
-    R acc[NR][6];
-#pragma unroll
-    for (int j = 0; j < NR; ++j)
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-            acc[j][k] = 0;
-    if (valid) {
-        const SiteCoord sc = site_coords(a.g, s, a.dst_parity);
-        hop_fwd<A, LF, NR, 0, 1>(a, a.fat, s, sc, src, acc);
-        hop_fwd<A, LL, NR, 0, 3>(a, a.lng, s, sc, src, acc);
-        hop_fwd<A, LF, NR, 1, 1>(a, a.fat, s, sc, src, acc);
-        hop_fwd<A, LL, NR, 1, 3>(a, a.lng, s, sc, src, acc);
-        hop_fwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
-        hop_fwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
-        hop_fwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
-        hop_fwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
-        hop_bwd<A, LF, NR, 0, 1>(a, a.fat, s, sc, src, acc);
-        hop_bwd<A, LL, NR, 0, 3>(a, a.lng, s, sc, src, acc);
-        hop_bwd<A, LF, NR, 1, 1>(a, a.fat, s, sc, src, acc);
-        hop_bwd<A, LL, NR, 1, 3>(a, a.lng, s, sc, src, acc);
-        hop_bwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
-        hop_bwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
-        hop_bwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
-        hop_bwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
-    }
     if constexpr (EPI == EPI_STORE) {
         if (valid) {
 #pragma unroll
@@ -163,6 +152,186 @@ __global__ void __launch_bounds__(NT) k_dslash(const DslashArgs<LF, LL, SR> a)
     }
 }
 
+// Per right-hand-side base pointers of a workgroup's block of NR systems, resolved once (uniform
+// over the block); padding slots beyond the nact live ones repeat the last live system
+template <int NR, class A>
+__device__ __forceinline__ void dslash_rhs(const A &a, int j0, int nact, int (&rr)[NR], typename A::V2 *(&dst)[NR],
+                                           const typename A::V2 *(&src)[NR], const typename A::V2 *(&aux)[NR], bool with_aux)
+{
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const int jj = (j0 + j < nact) ? j0 + j : nact - 1;
+        rr[j] = a.rlist ? a.rlist[jj] : jj;
+        src[j] = a.src.p + a.src.base(rr[j]);
+        dst[j] = a.dst.p + a.dst.base(rr[j]);
+        aux[j] = with_aux ? a.aux.p + a.aux.base(rr[j]) : nullptr;
+    }
+}
+
+template <class LF, class LL, typename SR, int NR, int EPI, int NT>
+__global__ void __launch_bounds__(NT) k_dslash(const DslashArgs<LF, LL, SR> a)
+{
+    using A = DslashArgs<LF, LL, SR>;
+    using R = typename A::R;
+    using V2 = c2_t<SR>;
+    __shared__ double lds[NT / kWave];
+    if (a.run != nullptr && *a.run == 0)
+        return;  // uniform over the grid
+    const int nact = a.nact ? *a.nact : a.nrhs;
+    const int j0 = blockIdx.y * NR;
+    if (j0 >= nact)
+        return;  // uniform over the block
+    int rr[NR];
+    const V2 *src[NR];
+    V2 *dst[NR];
+    const V2 *aux[NR];
+    dslash_rhs<NR>(a, j0, nact, rr, dst, src, aux, EPI == EPI_AXMY_DOT);
+    bool valid;
+    const uint32_t s = tile_site<NT>(a.g, blockIdx.x, gridDim.x, threadIdx.x, valid);
+    R acc[NR][6];
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            acc[j][k] = 0;
+    if (valid) {
+        const SiteCoord sc = site_coords(a.g, s, a.dst_parity);
+        hop_fwd<A, LF, NR, 0, 1>(a, a.fat, s, sc, src, acc);
+        hop_fwd<A, LL, NR, 0, 3>(a, a.lng, s, sc, src, acc);
+        hop_fwd<A, LF, NR, 1, 1>(a, a.fat, s, sc, src, acc);
+        hop_fwd<A, LL, NR, 1, 3>(a, a.lng, s, sc, src, acc);
+        hop_fwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
+        hop_fwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
+        hop_fwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
+        hop_fwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
+        hop_bwd<A, LF, NR, 0, 1>(a, a.fat, s, sc, src, acc);
+        hop_bwd<A, LL, NR, 0, 3>(a, a.lng, s, sc, src, acc);
+        hop_bwd<A, LF, NR, 1, 1>(a, a.fat, s, sc, src, acc);
+        hop_bwd<A, LL, NR, 1, 3>(a, a.lng, s, sc, src, acc);
+        hop_bwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
+        hop_bwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
+        hop_bwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
+        hop_bwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
+    }
+    dslash_epilogue<LF, SR, NR, EPI, NT>(a, acc, dst, aux, rr, j0, nact, s, valid, lds);
+}
+
+// The batched hopping term with the x and y neighbours read from LDS: the workgroup (256 sites,
+// whole x-rows of one (x, y) plane, stage_rows()) first copies its rows and their +-3
+// y-neighbour rows of every source into LDS; the z and t neighbours come from global memory.
+// Used for NR >= 2, where every staged row serves several right-hand sides' gathers: measured at
+// 32^4 with FP32 links, 4 right-hand sides 130 -> 116 us, 2 and 8 about 4 % faster (1 would be
+// a little slower). Results are bitwise those of k_dslash.
+template <class LF, class LL, typename SR, int NR, int EPI>
+__global__ void __launch_bounds__(256) k_dslash_staged(const DslashArgs<LF, LL, SR> a)
+{
+    using A = DslashArgs<LF, LL, SR>;
+    using R = typename A::R;
+    using V2 = c2_t<SR>;
+    constexpr int NT = 256, XH = kStageLxh, KR = NT + 6 * XH;
+    __shared__ __align__(16) V2 st[NR][3][KR];
+    __shared__ double lds[NT / kWave];
+    if (a.run != nullptr && *a.run == 0)
+        return;  // uniform over the grid
+    const int nact = a.nact ? *a.nact : a.nrhs;
+    const int j0 = blockIdx.y * NR;
+    if (j0 >= nact)
+        return;  // uniform over the block
+    int rr[NR];
+    const V2 *src[NR];
+    V2 *dst[NR];
+    const V2 *aux[NR];
+    dslash_rhs<NR>(a, j0, nact, rr, dst, src, aux, EPI == EPI_AXMY_DOT);
+    const uint32_t Vh = a.g.Vh;
+    // true for every thread where this kernel is launched (tile order); the hops and the epilogue
+    // stay under it as in k_dslash: with the hops outside, the compiler moved them into the
+    // epilogue's branches and used twice the registers, with spills
+    bool valid;
+    const uint32_t s = tile_site<NT>(a.g, blockIdx.x, gridDim.x, threadIdx.x, valid);
+    const SiteCoord sc = site_coords(a.g, s, a.dst_parity);
+    const int yl = (int)threadIdx.x / XH;
+    const int y0 = sc.c[1] - yl;  // uniform over the workgroup
+    const uint32_t pbase = s - threadIdx.x - (uint32_t)(XH * y0);
+    const int L1 = a.g.L[1];
+    // rows are XH entries of one colour plane, so two FP32 entries (16 bytes) load together when
+    // every source is 16-byte aligned (Vh even: so are its colour planes and rows)
+    bool wide = sizeof(V2) == 8 && (Vh % 2u) == 0;
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+        wide = wide && ((reinterpret_cast<uintptr_t>(src[j]) & 15u) == 0);
+    if (wide) {
+        constexpr int KH = KR / 2;
+        for (int e = threadIdx.x; e < NR * 3 * KH; e += NT) {
+            const int jc = e / KH, kk = 2 * (e - jc * KH), j = jc / 3, c = jc - 3 * j;
+            const int row = kk / XH, xh = kk - row * XH;
+            int ys = y0 - 3 + row;
+            ys = ys < 0 ? ys + L1 : (ys >= L1 ? ys - L1 : ys);
+            const uint32_t n = pbase + (uint32_t)(XH * ys + xh);
+            *reinterpret_cast<kq_v4f *>(&st[j][c][kk]) =
+                *reinterpret_cast<const kq_v4f *>(a.src.p + a.src.base(rr[j]) + (size_t)c * Vh + n);
+        }
+    } else {
+        for (int e = threadIdx.x; e < NR * 3 * KR; e += NT) {
+            const int jc = e / KR, kk = e - jc * KR, j = jc / 3, c = jc - 3 * j;
+            const int row = kk / XH, xh = kk - row * XH;
+            int ys = y0 - 3 + row;
+            ys = ys < 0 ? ys + L1 : (ys >= L1 ? ys - L1 : ys);
+            const uint32_t n = pbase + (uint32_t)(XH * ys + xh);
+            st[j][c][kk] = a.src.p[a.src.base(rr[j]) + (size_t)c * Vh + n];
+        }
+    }
+    __syncthreads();
+    R acc[NR][6];
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            acc[j][k] = 0;
+    if (valid) {
+        hop_staged<A, LF, NR, 0, 1, false, KR>(a, a.fat, s, sc, yl, st, acc);
+        hop_staged<A, LL, NR, 0, 3, false, KR>(a, a.lng, s, sc, yl, st, acc);
+        hop_staged<A, LF, NR, 1, 1, false, KR>(a, a.fat, s, sc, yl, st, acc);
+        hop_staged<A, LL, NR, 1, 3, false, KR>(a, a.lng, s, sc, yl, st, acc);
+        hop_fwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
+        hop_fwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
+        hop_fwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
+        hop_fwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
+        hop_staged<A, LF, NR, 0, 1, true, KR>(a, a.fat, s, sc, yl, st, acc);
+        hop_staged<A, LL, NR, 0, 3, true, KR>(a, a.lng, s, sc, yl, st, acc);
+        hop_staged<A, LF, NR, 1, 1, true, KR>(a, a.fat, s, sc, yl, st, acc);
+        hop_staged<A, LL, NR, 1, 3, true, KR>(a, a.lng, s, sc, yl, st, acc);
+        hop_bwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
+        hop_bwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
+        hop_bwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
+        hop_bwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
+    }
+    dslash_epilogue<LF, SR, NR, EPI, NT>(a, acc, dst, aux, rr, j0, nact, s, valid, lds);
+}
+
+// LDS of a workgroup the device allows
+static int lds_block_bytes()
+{
+    static const int lds_block = [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 65536;
+        return v;
+    }();
+    return lds_block;
+}
+
+// KQ_STAGE=0 turns the LDS row staging of the batched hopping terms off (a tuning switch;
+// results are bitwise the same either way)
+inline bool stage_enabled()
+{
+    static const bool on = [] {
+        const char *e = getenv("KQ_STAGE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 // dynamic LDS bytes that let at most `cap` waves per SIMD (4 SIMDs per CU) be resident
 static size_t cap_lds_bytes(int cap, int nt)
 {
@@ -175,13 +344,7 @@ static size_t cap_lds_bytes(int cap, int nt)
             v = 65536;
         return v;
     }();
-    static const int lds_block = [] {
-        int dev = 0, v = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
-            v = 65536;
-        return v;
-    }();
+    const int lds_block = lds_block_bytes();
     const int blocks = std::max(1, cap * 4 / std::max(1, nt / kWave));
     // a little under lds_cu / blocks, so that `blocks` fit and blocks + 1 do not
     const int bytes = lds_cu / blocks - 256;
@@ -194,6 +357,19 @@ static void launch_nt(const DslashArgs<LF, LL, SR> &a, int nt, int cap, hipStrea
     const int nact_max = a.nrhs;
     dim3 grid((a.g.Vh + nt - 1) / nt, (nact_max + NR - 1) / NR);
     const size_t lds = cap_lds_bytes(cap, nt);
+    if constexpr (NR >= 2) {
+        // staged rows (static LDS of the kernel): NR * 3 colour entries for each of the
+        // nt + 6 Lx/2 sites of the workgroup's rows and their +-3 y-neighbour rows; an occupancy
+        // cap adds dynamic LDS only beyond that
+        const size_t sb = (size_t)NR * 3 * (256 + 6 * kStageLxh) * sizeof(c2_t<SR>);
+        if (nt == 256 && a.g.lxh == kStageLxh && stage_enabled() && stage_rows(a.g, nt) > 0 &&
+            sb + 1024 <= (size_t)lds_block_bytes()) {
+            hipLaunchKernelGGL((k_dslash_staged<LF, LL, SR, NR, EPI>), grid, dim3(256), lds > sb ? lds - sb : 0,
+                               st, a);
+            KQ_HIP_CHECK(hipGetLastError());
+            return;
+        }
+    }
     switch (nt) {
     case 64:
         hipLaunchKernelGGL((k_dslash<LF, LL, SR, NR, EPI, 64>), grid, dim3(64), lds, st, a);

@@ -83,6 +83,31 @@ def test_dslash_batched_equals_single(gpu, mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("lat", [(32, 16, 8, 8), (16, 32, 8, 8), (32, 32, 4, 8)])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+def test_dslash_batched_staged_rows(gpu, lat, mode):
+    """Lattices whose 256-thread workgroups cover whole x-rows of one (x, y) plane, where the
+    batched terms (NR >= 2) stage the rows and their +-3 y-neighbour rows in LDS: one workgroup
+    per plane with more staged rows than the plane has (the y wrap repeats rows), and two
+    workgroups per plane. Same results as single right-hand sides (bitwise the same arithmetic)
+    and as the oracle."""
+    print("case", lat, mode, flush=True)
+    fat, lng = _setup(gpu, lat, gpu.KUQCD_DOUBLE)
+    nrhs = 5
+    src = np.stack([gpu.gen_source(lat, 0, seed=200 + r) for r in range(nrhs)])
+    single = np.empty_like(src)
+    for r in range(nrhs):
+        assert gpu.dslash(np.ascontiguousarray(src[r]), single[r], 1, mode) == 0
+    tol = {1: 1e-13, 2: 1e-5, 3: 3e-3, 4: 5e-4}[mode]
+    for r in (0, nrhs - 1):
+        assert rel_err(single[r], oracle.dslash(fat, lng, src[r], lat, 1)) <= tol
+    for nr in ([2] if mode == 1 else [2, 3, 4, 8]):
+        out = np.empty_like(src)
+        assert gpu.dslash_batched(src, out, 1, mode, nr, 256) == 0, gpu.last_error()
+        assert rel_err(out, single) <= (1e-15 if mode == 1 else 1e-6), nr
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("parity", [0, 1])
 def test_mdaggm_fp64(gpu, parity):
     lat = (8, 8, 8, 8)
