@@ -312,7 +312,11 @@ def main() -> int:
     pmc = ROOT / "profiles" / "dslash_32x32x32x32_pmc.txt"
     k64 = "k_dslash<kq::LinkD, kq::LinkD14, double, 1" if d14 else "k_dslash<kq::LinkD, kq::LinkD, double, 1"
     traffic = pmc_traffic(pmc, k64) if lat == (32, 32, 32, 32) else None
-    traffic_x4 = pmc_traffic(pmc, "k_dslash<kq::LinkF, kq::LinkF12, float, 4") if lat == (32, 32, 32, 32) else None
+    # with 256-thread workgroups on Lx = 32 lattices the batched term stages its x/y neighbour rows in
+    # LDS (k_dslash_staged); KQ_STAGE=0 switches that off
+    staged_x4 = r4.nt == 256 and lat[0] == 32 and os.environ.get("KQ_STAGE", "1") != "0"
+    k_x4 = "k_dslash_staged<kq::LinkF, kq::LinkF12, float, 4" if staged_x4 else "k_dslash<kq::LinkF, kq::LinkF12, float, 4"
+    traffic_x4 = pmc_traffic(pmc, k_x4) if lat == (32, 32, 32, 32) else None
 
     # CPU baseline: the oracle CGM (the reference's algorithm, FP64) solving the same problem to
     # completion on the host cores; its solution also gives each shift's error
@@ -398,7 +402,8 @@ def main() -> int:
         "roofline_fp32_x4": {"achieved": round(r4.gbs, 1), "frac": round(r4.hbm_frac, 4),
                              "us_per_launch": round(r4.ms * 1e3, 2), "bytes_per_site": round(r4.bytes / Vh, 1),
                              "nr": r4.nr, "nt": r4.nt, "traffic": traffic_x4,
-                             "kernel": "k_dslash<LinkF,LinkF12,float,4> (the mixed solve's dominant kernel)"},
+                             "kernel": (f"{'k_dslash_staged' if staged_x4 else 'k_dslash'}<LinkF,LinkF12,float,4> "
+                                        "(the mixed solve's dominant kernel)")},
         "roofline_fp16": {"achieved": round(rl["fp16"].gbs, 1), "frac": round(rl["fp16"].hbm_frac, 4),
                           "us_per_launch": round(rl["fp16"].ms * 1e3, 2),
                           "bytes_per_site": round(rl["fp16"].bytes / Vh, 1)},
