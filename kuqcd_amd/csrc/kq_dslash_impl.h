@@ -95,6 +95,86 @@ __device__ __forceinline__ void hop_staged(const A &a, const LK &lk, uint32_t s,
     }
 }
 
+// The staged kernel's 16 hops as one chain in which each hop's link words are requested before
+// the previous hop's arithmetic (a one-hop prefetch; hop I: 0-3 x/y forward from LDS, 4-7 z/t
+// forward from global memory, 8-11 and 12-15 the same backward; even I fat, odd I long links).
+// A scheduling fence after each prefetch keeps the compiler from sinking it back. Measured at 32^4
+// with FP32 links: 4 right-hand sides -2 %, 8 -8 %, 2 and 3 within 1 % (so used from 4 on, FP32
+// links only: the FP64 link words would take 28-36 more registers).
+template <int I> struct StagedHop {
+    static constexpr int kMu = (I % 8) / 2;  // 0, 1 staged (x, y); 2, 3 global (z, t)
+    static constexpr bool kBwd = I >= 8, kLong = (I % 2) == 1;
+    static constexpr int kH = kLong ? 3 : 1;
+};
+
+template <class A, int I>
+__device__ __forceinline__ auto staged_fetch(const A &a, uint32_t s, const SiteCoord &sc, uint32_t &n)
+{
+    using H = StagedHop<I>;
+    n = nb_index<H::kMu>(a.g, s, sc, H::kBwd ? -H::kH : H::kH);
+    const uint32_t ls = H::kBwd ? n : s;
+    const int q = H::kBwd ? 1 - a.dst_parity : a.dst_parity;
+    if constexpr (H::kLong)
+        return a.lng.fetch(q, H::kMu, ls);
+    else
+        return a.fat.fetch(q, H::kMu, ls);
+}
+
+template <class A, int NR, int KR, int I, class RW>
+__device__ __forceinline__ void staged_chain(const A &a, uint32_t s, const SiteCoord &sc, int yl,
+                                             const typename A::V2 (*st)[3][KR], const typename A::V2 *const src[NR],
+                                             typename A::R acc[NR][6], const RW &cur, uint32_t n)
+{
+    using H = StagedHop<I>;
+    constexpr int XH = kStageLxh;
+    typename A::V2 v[NR][3];
+    if constexpr (H::kMu < 2) {
+        const int h = H::kBwd ? -H::kH : H::kH;
+        const int k = H::kMu == 0 ? (yl + 3) * XH + (wrap_add(sc.c[0], h, a.g.L[0]) >> 1)
+                                  : (yl + 3 + h) * XH + (sc.c[0] >> 1);
+#pragma unroll
+        for (int j = 0; j < NR; ++j)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                v[j][c] = st[j][c][k];
+    } else {
+#pragma unroll
+        for (int j = 0; j < NR; ++j)
+            gather(src[j], n, a.g.Vh, v[j]);
+    }
+    uint32_t n2 = 0;
+    if constexpr (I < 15) {
+        const auto nxt = staged_fetch<A, I + 1>(a, s, sc, n2);
+        __builtin_amdgcn_sched_barrier(0);
+        c2_t<typename A::R> U[9];
+        if constexpr (H::kLong)
+            a.lng.unpack(cur, U);
+        else
+            a.fat.unpack(cur, U);
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            if constexpr (H::kBwd)
+                udmv_sub(U, v[j], acc[j]);
+            else
+                umv_add(U, v[j], acc[j]);
+        }
+        staged_chain<A, NR, KR, I + 1>(a, s, sc, yl, st, src, acc, nxt, n2);
+    } else {
+        c2_t<typename A::R> U[9];
+        if constexpr (H::kLong)
+            a.lng.unpack(cur, U);
+        else
+            a.fat.unpack(cur, U);
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            if constexpr (H::kBwd)
+                udmv_sub(U, v[j], acc[j]);
+            else
+                umv_add(U, v[j], acc[j]);
+        }
+    }
+}
+
 // output store: streaming for the FP64 and FP32 link formats (measured 5% and 4%
 // faster at 32^4), plain for the 16-bit one (3% slower with streaming stores). Also
 // streaming where the next hopping term gathers the output (MdaggM's intermediate):
@@ -288,22 +368,28 @@ __global__ void __launch_bounds__(256) k_dslash_staged(const DslashArgs<LF, LL, 
         for (int k = 0; k < 6; ++k)
             acc[j][k] = 0;
     if (valid) {
-        hop_staged<A, LF, NR, 0, 1, false, KR>(a, a.fat, s, sc, yl, st, acc);
-        hop_staged<A, LL, NR, 0, 3, false, KR>(a, a.lng, s, sc, yl, st, acc);
-        hop_staged<A, LF, NR, 1, 1, false, KR>(a, a.fat, s, sc, yl, st, acc);
-        hop_staged<A, LL, NR, 1, 3, false, KR>(a, a.lng, s, sc, yl, st, acc);
-        hop_fwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
-        hop_fwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
-        hop_fwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
-        hop_fwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
-        hop_staged<A, LF, NR, 0, 1, true, KR>(a, a.fat, s, sc, yl, st, acc);
-        hop_staged<A, LL, NR, 0, 3, true, KR>(a, a.lng, s, sc, yl, st, acc);
-        hop_staged<A, LF, NR, 1, 1, true, KR>(a, a.fat, s, sc, yl, st, acc);
-        hop_staged<A, LL, NR, 1, 3, true, KR>(a, a.lng, s, sc, yl, st, acc);
-        hop_bwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
-        hop_bwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
-        hop_bwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
-        hop_bwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
+        if constexpr (NR >= 4 && std::is_same<R, float>::value) {
+            uint32_t n0;
+            const auto r0 = staged_fetch<A, 0>(a, s, sc, n0);
+            staged_chain<A, NR, KR, 0>(a, s, sc, yl, st, src, acc, r0, n0);
+        } else {
+            hop_staged<A, LF, NR, 0, 1, false, KR>(a, a.fat, s, sc, yl, st, acc);
+            hop_staged<A, LL, NR, 0, 3, false, KR>(a, a.lng, s, sc, yl, st, acc);
+            hop_staged<A, LF, NR, 1, 1, false, KR>(a, a.fat, s, sc, yl, st, acc);
+            hop_staged<A, LL, NR, 1, 3, false, KR>(a, a.lng, s, sc, yl, st, acc);
+            hop_fwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
+            hop_fwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
+            hop_fwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
+            hop_fwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
+            hop_staged<A, LF, NR, 0, 1, true, KR>(a, a.fat, s, sc, yl, st, acc);
+            hop_staged<A, LL, NR, 0, 3, true, KR>(a, a.lng, s, sc, yl, st, acc);
+            hop_staged<A, LF, NR, 1, 1, true, KR>(a, a.fat, s, sc, yl, st, acc);
+            hop_staged<A, LL, NR, 1, 3, true, KR>(a, a.lng, s, sc, yl, st, acc);
+            hop_bwd<A, LF, NR, 2, 1>(a, a.fat, s, sc, src, acc);
+            hop_bwd<A, LL, NR, 2, 3>(a, a.lng, s, sc, src, acc);
+            hop_bwd<A, LF, NR, 3, 1>(a, a.fat, s, sc, src, acc);
+            hop_bwd<A, LL, NR, 3, 3>(a, a.lng, s, sc, src, acc);
+        }
     }
     dslash_epilogue<LF, SR, NR, EPI, NT>(a, acc, dst, aux, rr, j0, nact, s, valid, lds);
 }

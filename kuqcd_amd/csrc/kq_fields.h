@@ -80,15 +80,26 @@ struct LinkD {  // FP64, 18 reals: 9 chunks of 16 B (one complex each)
     static constexpr size_t kBlock = 64 * kBytes;
     const char *p;
     uint32_t nb64;
-    __device__ inline void load(int q, int mu, uint32_t s, double2 u[9]) const
+    // a link's stored words (fetch) and the matrix they encode (unpack); load = unpack(fetch)
+    struct Raw {
+        kq_v2d c[9];
+    };
+    __device__ inline Raw fetch(int q, int mu, uint32_t s) const
     {
+        Raw r;
         const kq_v2d *b = reinterpret_cast<const kq_v2d *>(p + link_block_offset(q, mu, s, nb64, kBlock)) + (s & 63);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const kq_v2d v = ld_nt(b + 64 * k);
-            u[k] = mk<double2>(v.x, v.y);
-        }
+        for (int k = 0; k < 9; ++k)
+            r.c[k] = ld_nt(b + 64 * k);
+        return r;
     }
+    __device__ inline void unpack(const Raw &r, double2 u[9]) const
+    {
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            u[k] = mk<double2>(r.c[k].x, r.c[k].y);
+    }
+    __device__ inline void load(int q, int mu, uint32_t s, double2 u[9]) const { unpack(fetch(q, mu, s), u); }
 };
 
 // FP64 long links as 14 reals: rows 0 and 1 of an SU(3) matrix V and a complex scale w, U = w V
@@ -101,16 +112,26 @@ struct LinkD14 {
     static constexpr size_t kBlock = 64 * kBytes;
     const char *p;
     uint32_t nb64;
-    __device__ inline void load(int q, int mu, uint32_t s, double2 u[9]) const
+    struct Raw {
+        kq_v2d c[7];
+    };
+    __device__ inline Raw fetch(int q, int mu, uint32_t s) const
     {
+        Raw r;
         const kq_v2d *b = reinterpret_cast<const kq_v2d *>(p + link_block_offset(q, mu, s, nb64, kBlock)) + (s & 63);
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            r.c[k] = ld_nt(b + 64 * k);
+        return r;
+    }
+    __device__ inline void load(int q, int mu, uint32_t s, double2 u[9]) const { unpack(fetch(q, mu, s), u); }
+    __device__ inline void unpack(const Raw &r, double2 u[9]) const
+    {
         double2 v[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const kq_v2d c = ld_nt(b + 64 * k);
-            v[k] = mk<double2>(c.x, c.y);
-        }
-        const kq_v2d w = ld_nt(b + 64 * 6);
+        for (int k = 0; k < 6; ++k)
+            v[k] = mk<double2>(r.c[k].x, r.c[k].y);
+        const kq_v2d w = r.c[6];
         // row 2 of V = conj(row0 x row1), then U = w V
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -132,19 +153,31 @@ struct LinkF {  // FP32, 18 reals: 4 chunks of 16 B (complex 0..7) + 1 of 8 B (c
     static constexpr size_t kBlock = 64 * kBytes;
     const char *p;
     uint32_t nb64;
-    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
+    struct Raw {
+        kq_v4f c[4];
+        kq_v2f d;
+    };
+    __device__ inline Raw fetch(int q, int mu, uint32_t s) const
     {
+        Raw r;
         const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
         const uint32_t l = s & 63;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const kq_v4f v = ld_nt(reinterpret_cast<const kq_v4f *>(b + 1024 * c) + l);
-            u[2 * c] = mk<float2>(v.x, v.y);
-            u[2 * c + 1] = mk<float2>(v.z, v.w);
-        }
-        const kq_v2f w = ld_nt(reinterpret_cast<const kq_v2f *>(b + 4096) + l);
-        u[8] = mk<float2>(w.x, w.y);
+        for (int c = 0; c < 4; ++c)
+            r.c[c] = ld_nt(reinterpret_cast<const kq_v4f *>(b + 1024 * c) + l);
+        r.d = ld_nt(reinterpret_cast<const kq_v2f *>(b + 4096) + l);
+        return r;
     }
+    __device__ inline void unpack(const Raw &r, float2 u[9]) const
+    {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            u[2 * c] = mk<float2>(r.c[c].x, r.c[c].y);
+            u[2 * c + 1] = mk<float2>(r.c[c].z, r.c[c].w);
+        }
+        u[8] = mk<float2>(r.d.x, r.d.y);
+    }
+    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const { unpack(fetch(q, mu, s), u); }
 };
 
 // 16-bit links, 18 reals: each real is a 16-bit fixed-point integer q with
@@ -157,13 +190,25 @@ struct LinkH {
     static constexpr size_t kBlock = 64 * kBytes;
     const char *p;
     uint32_t nb64;
-    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
+    struct Raw {
+        kq_v4u c0, c1;
+        kq_v2u c2;
+    };
+    __device__ inline Raw fetch(int q, int mu, uint32_t s) const
     {
+        Raw r;
         const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
         const uint32_t l = s & 63;
-        const kq_v4u c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
-        const kq_v4u c1 = ld_nt(reinterpret_cast<const kq_v4u *>(b + 1024) + l);
-        const kq_v2u c2 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 2048) + l);
+        r.c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
+        r.c1 = ld_nt(reinterpret_cast<const kq_v4u *>(b + 1024) + l);
+        r.c2 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 2048) + l);
+        return r;
+    }
+    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const { unpack(fetch(q, mu, s), u); }
+    __device__ inline void unpack(const Raw &r, float2 u[9]) const
+    {
+        const kq_v4u c0 = r.c0, c1 = r.c1;
+        const kq_v2u c2 = r.c2;
         const float a = __uint_as_float(c2.y);
         const unsigned int h[9] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x};
 #pragma unroll
@@ -183,13 +228,27 @@ struct LinkH12 {
     static constexpr size_t kBlock = 64 * kBytes;
     const char *p;
     uint32_t nb64;
-    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
+    struct Raw {
+        kq_v4u c0;
+        kq_v2u c1;
+        float a;
+    };
+    __device__ inline Raw fetch(int q, int mu, uint32_t s) const
     {
+        Raw r;
         const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
         const uint32_t l = s & 63;
-        const kq_v4u c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
-        const kq_v2u c1 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 1024) + l);
-        const float a = ld_nt(reinterpret_cast<const float *>(b + 1536) + l);
+        r.c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
+        r.c1 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 1024) + l);
+        r.a = ld_nt(reinterpret_cast<const float *>(b + 1536) + l);
+        return r;
+    }
+    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const { unpack(fetch(q, mu, s), u); }
+    __device__ inline void unpack(const Raw &r, float2 u[9]) const
+    {
+        const kq_v4u c0 = r.c0;
+        const kq_v2u c1 = r.c1;
+        const float a = r.a;
         constexpr float kQ = 1.0f / 32767.0f;
         float2 v[6];
         v[0] = q2f(c0.x);
@@ -229,18 +288,31 @@ struct LinkF12 {
     static constexpr size_t kBlock = 64 * kBytes;
     const char *p;
     uint32_t nb64;
-    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
+    struct Raw {
+        kq_v4f c[3];
+        float a;
+    };
+    __device__ inline Raw fetch(int q, int mu, uint32_t s) const
     {
+        Raw r;
         const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
         const uint32_t l = s & 63;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            r.c[c] = ld_nt(reinterpret_cast<const kq_v4f *>(b + 1024 * c) + l);
+        r.a = ld_nt(reinterpret_cast<const float *>(b + 3072) + l);
+        return r;
+    }
+    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const { unpack(fetch(q, mu, s), u); }
+    __device__ inline void unpack(const Raw &r, float2 u[9]) const
+    {
         float2 v[6];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const kq_v4f w = ld_nt(reinterpret_cast<const kq_v4f *>(b + 1024 * c) + l);
-            v[2 * c] = mk<float2>(w.x, w.y);
-            v[2 * c + 1] = mk<float2>(w.z, w.w);
+            v[2 * c] = mk<float2>(r.c[c].x, r.c[c].y);
+            v[2 * c + 1] = mk<float2>(r.c[c].z, r.c[c].w);
         }
-        const float a = ld_nt(reinterpret_cast<const float *>(b + 3072) + l);
+        const float a = r.a;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int c1i = (c + 1) % 3, c2i = (c + 2) % 3;
@@ -280,18 +352,31 @@ struct LinkF12C {
     static constexpr size_t kBlock = 64 * kBytes;
     const char *p;
     uint32_t nb64;
-    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
+    struct Raw {
+        kq_v4f c[3];
+        kq_v2f w;
+    };
+    __device__ inline Raw fetch(int q, int mu, uint32_t s) const
     {
+        Raw r;
         const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
         const uint32_t l = s & 63;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            r.c[c] = ld_nt(reinterpret_cast<const kq_v4f *>(b + 1024 * c) + l);
+        r.w = ld_nt(reinterpret_cast<const kq_v2f *>(b + 3072) + l);
+        return r;
+    }
+    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const { unpack(fetch(q, mu, s), u); }
+    __device__ inline void unpack(const Raw &r, float2 u[9]) const
+    {
         float2 v[6];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const kq_v4f w4 = ld_nt(reinterpret_cast<const kq_v4f *>(b + 1024 * c) + l);
-            v[2 * c] = mk<float2>(w4.x, w4.y);
-            v[2 * c + 1] = mk<float2>(w4.z, w4.w);
+            v[2 * c] = mk<float2>(r.c[c].x, r.c[c].y);
+            v[2 * c + 1] = mk<float2>(r.c[c].z, r.c[c].w);
         }
-        const kq_v2f w = ld_nt(reinterpret_cast<const kq_v2f *>(b + 3072) + l);
+        const kq_v2f w = r.w;
         su3_row2_scaled<float>(v, w.x, w.y, u);
 #pragma unroll
         for (int k = 0; k < 6; ++k)
@@ -307,13 +392,27 @@ struct LinkH12C {
     static constexpr size_t kBlock = 64 * kBytes;
     const char *p;
     uint32_t nb64;
-    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const
+    struct Raw {
+        kq_v4u c0;
+        kq_v2u c1;
+        kq_v2f w;
+    };
+    __device__ inline Raw fetch(int q, int mu, uint32_t s) const
     {
+        Raw r;
         const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
         const uint32_t l = s & 63;
-        const kq_v4u c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
-        const kq_v2u c1 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 1024) + l);
-        const kq_v2f w = ld_nt(reinterpret_cast<const kq_v2f *>(b + 1536) + l);
+        r.c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
+        r.c1 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 1024) + l);
+        r.w = ld_nt(reinterpret_cast<const kq_v2f *>(b + 1536) + l);
+        return r;
+    }
+    __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const { unpack(fetch(q, mu, s), u); }
+    __device__ inline void unpack(const Raw &r, float2 u[9]) const
+    {
+        const kq_v4u c0 = r.c0;
+        const kq_v2u c1 = r.c1;
+        const kq_v2f w = r.w;
         constexpr float kQ = 1.0f / 32767.0f;
         const unsigned int h[6] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y};
         float2 v[6];
