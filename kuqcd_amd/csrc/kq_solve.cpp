@@ -42,12 +42,30 @@ static void print_iters(int iter, int max_iter)
 
 // occupancy caps of the hopping term for every blocking the runner may use, and the
 // FP64 term of the reliable updates (block size and caps)
+// KQ_CAP=<waves per SIMD> (0..8) replaces every autotuned occupancy cap of the solvers' hopping
+// terms (a tuning switch: results are bitwise the same, only the resident waves differ)
+static int cap_override()
+{
+    static const int v = [] {
+        const char *e = getenv("KQ_CAP");
+        if (e == nullptr)
+            return -1;
+        const int n = atoi(e);
+        return (n >= 0 && n <= 8) ? n : -1;
+    }();
+    return v;
+}
+
 static void set_caps(RunnerTuning &t, int slot, int mode, bool reliable)
 {
     const bool dbl = mode == KQ_MODE_FP64 || (mode == KQ_MODE_REFERENCE && ctx().fat[slot].prec == KUQCD_DOUBLE);
     static const int kNrs[5] = {1, 2, 3, 4, 8};
     for (int i = 0; i < 5; ++i)
         t.cap[i] = (dbl && kNrs[i] > 2) ? 0 : tuned_cap(slot, mode, kNrs[i], t.nt);
+    const int ov = cap_override();
+    if (ov >= 0)
+        for (int i = 0; i < 5; ++i)
+            t.cap[i] = ov;
     if (reliable) {
         t.nt64 = tuned_launch(slot, KQ_MODE_FP64, 1).second;
         // KQ_RELIABLE_NT=64|128|256 fixes the FP64 term's block size (to test the runner
@@ -59,6 +77,8 @@ static void set_caps(RunnerTuning &t, int slot, int mode, bool reliable)
         }
         t.cap64[0] = tuned_cap(slot, KQ_MODE_FP64, 1, t.nt64);
         t.cap64[1] = tuned_cap(slot, KQ_MODE_FP64, 2, t.nt64);
+        if (ov >= 0)
+            t.cap64[0] = t.cap64[1] = ov;
     }
 }
 
