@@ -50,4 +50,27 @@ inline uint32_t dslash_nblk(uint32_t Vh, int nt)
     return nb;
 }
 
+// Row length Lx/2 of the lattices whose batched hopping terms stage rows in LDS (Lx = 32, as at
+// 32^4): a compile-time row length gives the staged kernel a static LDS size
+constexpr int kStageLxh = 16;
+
+// KQ_STAGE=0 turns the LDS row staging of the batched hopping terms off (a tuning switch;
+// results are bitwise the same either way)
+inline bool stage_enabled()
+{
+    static const bool on = [] {
+        const char *e = getenv("KQ_STAGE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+// Whether launch_dslash runs the LDS-staged kernel for nr right-hand sides per thread and block
+// size nt on this lattice (its NR * 3 * 352 staged colour entries, at most 67.6 KB for eight
+// FP32 or 33.8 KB for two FP64 right-hand sides, fit a workgroup's LDS on gfx950)
+inline bool dslash_staged(const Geom &g, int nr, int nt)
+{
+    return nr >= 2 && nt == 256 && g.lxh == kStageLxh && stage_enabled() && stage_rows(g, nt) > 0;
+}
+
 }  // namespace kq

@@ -61,10 +61,6 @@ __device__ __forceinline__ void hop_bwd(const A &a, const LK &lk, uint32_t s, co
     }
 }
 
-// Row length Lx/2 of the lattices whose batched hopping terms stage rows in LDS (Lx = 32, as at
-// 32^4): a compile-time row length gives a static LDS size
-constexpr int kStageLxh = 16;
-
 // Hop along x (MU 0) or y (MU 1) of a workgroup whose source rows are staged in LDS:
 // st[j][c][k] holds colour c of right-hand side j at k = (y - y0 + 3) * Lx/2 + x/2 for the rows
 // y0 - 3 .. y0 + NT / (Lx/2) + 2 of the workgroup's plane (yl = y - y0). Same arithmetic, in the
@@ -407,17 +403,6 @@ static int lds_block_bytes()
     return lds_block;
 }
 
-// KQ_STAGE=0 turns the LDS row staging of the batched hopping terms off (a tuning switch;
-// results are bitwise the same either way)
-inline bool stage_enabled()
-{
-    static const bool on = [] {
-        const char *e = getenv("KQ_STAGE");
-        return !(e && e[0] == '0');
-    }();
-    return on;
-}
-
 // dynamic LDS bytes that let at most `cap` waves per SIMD (4 SIMDs per CU) be resident
 static size_t cap_lds_bytes(int cap, int nt)
 {
@@ -448,8 +433,7 @@ static void launch_nt(const DslashArgs<LF, LL, SR> &a, int nt, int cap, hipStrea
         // nt + 6 Lx/2 sites of the workgroup's rows and their +-3 y-neighbour rows; an occupancy
         // cap adds dynamic LDS only beyond that
         const size_t sb = (size_t)NR * 3 * (256 + 6 * kStageLxh) * sizeof(c2_t<SR>);
-        if (nt == 256 && a.g.lxh == kStageLxh && stage_enabled() && stage_rows(a.g, nt) > 0 &&
-            sb + 1024 <= (size_t)lds_block_bytes()) {
+        if (dslash_staged(a.g, NR, nt) && sb + 1024 <= (size_t)lds_block_bytes()) {
             hipLaunchKernelGGL((k_dslash_staged<LF, LL, SR, NR, EPI>), grid, dim3(256), lds > sb ? lds - sb : 0,
                                st, a);
             KQ_HIP_CHECK(hipGetLastError());

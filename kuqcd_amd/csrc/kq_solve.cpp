@@ -281,8 +281,10 @@ static std::vector<double> true_res2_t(int slot, LinkD fat, LL lng, int parity, 
     a.aux = xv;
     a.sigma = sig.as<double>();
     a.partial = part.as<double>();
-    // one vector per thread for the fused epilogue (its two-vector FP64 form is slower than two passes)
-    launch_dslash(a, EPI_AXMY_DOT, 1, nt, c.stream, tuned_cap(slot, KQ_MODE_FP64, 1, nt));
+    // the fused epilogue's term with two vectors per thread where it is the LDS-staged one, else
+    // one (the unstaged two-vector FP64 form is slower than two passes)
+    const int nr2 = (nr == 2 && dslash_staged(g, 2, nt)) ? 2 : 1;
+    launch_dslash(a, EPI_AXMY_DOT, nr2, nt, c.stream, tuned_cap(slot, KQ_MODE_FP64, nr2, nt));
     const uint32_t nb = blas_nblk(g.Vh);
     resid2_partial(b, ax.as<double>(), nl, list.as<int>(), g.Vh, part.as<double>(), nb, c.stream);
     const std::vector<double> all = reduce_host(part.as<double>(), nb, nk, c.stream);

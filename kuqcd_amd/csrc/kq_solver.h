@@ -601,9 +601,10 @@ private:
     int list_grid() const { return list_size_; }
 
     // ax64 = A_0 x_hi[.][0] for the n listed systems (FP64 links). The first hopping term reads
-    // the links once for two systems; the second, with the fused sigma x - D tmp and <x, Ax>
-    // epilogue, runs one system per thread: with two its FP64 registers cost more than a
-    // second pass over the links (478 us against 2 x 195 us at 32^4)
+    // the links once for two systems; so does the second, with the fused sigma x - D tmp and
+    // <x, Ax> epilogue, where the two-system term is the LDS-staged one (195 us against 2 x 173
+    // us at 32^4); elsewhere it runs one system per thread, since the unstaged two-system form
+    // costs more in FP64 registers than a second pass over the links (478 against 2 x 195 us)
     void apply_a64_x0(int n)
     {
         if (l14_)
@@ -635,7 +636,8 @@ private:
         a.aux = x0;
         a.sigma = sigma_.as<double>();
         a.partial = part_a_.as<double>();
-        launch_dslash(a, EPI_AXMY_DOT, 1, tune_.nt64, st_, tune_.cap64[0]);
+        const int nr2 = (nr == 2 && dslash_staged(g_, 2, tune_.nt64)) ? 2 : 1;
+        launch_dslash(a, EPI_AXMY_DOT, nr2, tune_.nt64, st_, tune_.cap64[nr2 - 1]);
     }
 
     void start_pending()
