@@ -107,6 +107,51 @@ def test_dslash_batched_staged_rows(gpu, lat, mode):
         assert rel_err(out, single) <= (1e-15 if mode == 1 else 1e-6), nr
 
 
+_UNSTAGED_CHILD = """
+import sys
+import numpy as np
+sys.path.insert(0, {root!r})
+import kuqcd_amd.api as A
+A.L.load()
+A.set_verbosity(0)
+assert A.kuqcd_init() == A.KUQCD_SUCCESS
+lat = {lat!r}
+fat, lng = A.gen_gauge(lat, kind=0, seed_fat=11, seed_long=12)
+A.load_gauge_pair(fat, lng, lat, prec=A.KUQCD_DOUBLE)
+src = np.stack([A.gen_source(lat, 0, seed=300 + r) for r in range({nrhs})])
+out = np.empty_like(src)
+assert A.dslash_batched(src, out, 1, {mode}, {nr}, 256) == 0, A.last_error()
+np.save({path!r}, out)
+A.kuqcd_gauge_free()
+A.kuqcd_end()
+"""
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,nr", [(1, 2), (2, 4), (2, 8), (4, 3)])
+def test_dslash_staged_rows_bitwise_equal_to_unstaged(gpu, tmp_path, mode, nr):
+    """The LDS-staged batched term (and its link prefetch for 4 or more FP32 right-hand sides)
+    performs the unstaged term's arithmetic in the same order: the outputs are bitwise equal to
+    those of a process with the staging switched off (KQ_STAGE=0)."""
+    import os
+    import subprocess
+    import sys
+    from conftest import ROOT
+
+    lat, nrhs = (32, 16, 8, 8), 5
+    print("case", mode, nr, flush=True)
+    _setup(gpu, lat, gpu.KUQCD_DOUBLE)
+    src = np.stack([gpu.gen_source(lat, 0, seed=300 + r) for r in range(nrhs)])
+    out = np.empty_like(src)
+    assert gpu.dslash_batched(src, out, 1, mode, nr, 256) == 0, gpu.last_error()
+    path = str(tmp_path / "unstaged.npy")
+    code = _UNSTAGED_CHILD.format(root=str(ROOT), lat=lat, nrhs=nrhs, mode=mode, nr=nr, path=path)
+    r = subprocess.run([sys.executable, "-c", code], env={**os.environ, "KQ_STAGE": "0"}, capture_output=True,
+                       text=True, timeout=180)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert np.array_equal(out, np.load(path))
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("parity", [0, 1])
 def test_mdaggm_fp64(gpu, parity):
