@@ -40,6 +40,7 @@ void ensure_device()
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
         throw std::runtime_error("no HIP device is visible");
     KQ_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    KQ_HIP_CHECK(hipStreamCreateWithFlags(&c.copy_stream, hipStreamNonBlocking));
     DevPool::get().set_stream(c.stream);
     (void)noalloc_scratch();  // the no-allocation entry point's scratch address on this device
     c.device_ready = true;
@@ -521,8 +522,10 @@ void kq_end(void)
 {
     try {
         Context &c = ctx();
-        if (c.device_ready)
+        if (c.device_ready) {
             (void)hipStreamSynchronize(c.stream);
+            (void)hipStreamSynchronize(c.copy_stream);
+        }
         learned_release();
         DevPool::get().trim();
     } catch (...) {

@@ -55,6 +55,8 @@ inline LinkH12C as_linkH12C(const DevBuf &b, uint32_t Vh) { return LinkH12C{b.as
 struct Context {
     bool device_ready = false;
     hipStream_t stream = nullptr;
+    // second stream (default priority) for device-to-host copies that overlap the solver's kernels
+    hipStream_t copy_stream = nullptr;
     int grid[4] = {1, 1, 1, 1};
     // slot order = reference dispatch order (kuqcd.cpp:415-435):
     // 0 float/BATCH, 1 double/BATCH, 2 float/NORMAL, 3 double/NORMAL

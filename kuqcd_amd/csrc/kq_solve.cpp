@@ -107,9 +107,35 @@ static KuQCDStatus solve_ref_t(int slot, LF fat, LL lng, void **x, void *b, KuQC
         sh[k] = (T)p.shifts[k];  // kuqcd.cpp:270-272
     const std::vector<double> tolk(ns, p.tol);
     run.add_system(ns, sh.data(), tolk.data(), p.max_iter, bdev.as<void>(), nullptr);
-    run.run();
+    // A shift the solver drops is final once the fold enqueued with its drop has run: its copy
+    // to the caller's array goes out on the copy stream while the next chunk of iterations runs
+    // (the host blocks in that copy, not the GPU), so the transfers of all but the last shifts
+    // overlap the solve.
+    std::vector<char> copied(ns, 0);
+    std::vector<std::pair<int, hipEvent_t>> ready;
+    DevBuf stage2(sizeof(T) * 6 * (size_t)g.Vh);
+    auto on_drop = [&](int, int k) {
+        hipEvent_t e;
+        KQ_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        KQ_HIP_CHECK(hipEventRecord(e, c.stream));
+        ready.emplace_back(k, e);
+    };
+    auto after_launch = [&] {
+        for (const auto &kv : ready) {
+            KQ_HIP_CHECK(hipStreamWaitEvent(c.copy_stream, kv.second, 0));
+            Vec<T> xv{reinterpret_cast<c2_t<T> *>(run.x_lo(0, kv.first)), g.Vh, n3};
+            download_vec<T, T>(xv, 0, x[kv.first], stage2, c.copy_stream);
+            (void)hipEventDestroy(kv.second);
+            copied[kv.first] = 1;
+        }
+        ready.clear();
+    };
+    run.run(on_drop, after_launch);
+    after_launch();
     const int iter = run.state(0).iter;
     for (int k = 0; k < ns; ++k) {
+        if (copied[k])
+            continue;
         Vec<T> xv{reinterpret_cast<c2_t<T> *>(run.x_lo(0, k)), g.Vh, n3};
         download_vec<T, T>(xv, 0, x[k], stage, c.stream);
     }

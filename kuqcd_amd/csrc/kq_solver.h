@@ -389,8 +389,11 @@ public:
 
     // Iterate until every registered system has stopped. on_drop(sys, k) is
     // called (between chunks) for each shift k >= 1 that a system drops; by
-    // then its solution is final in x_hi(sys, k) (reliable) or x_lo(sys, k).
-    void run(const DropFn &on_drop = nullptr)
+    // then its solution is final in x_hi(sys, k) (reliable) or x_lo(sys, k)
+    // once the work enqueued so far has run. after_launch() is called each time
+    // a chunk of iterations has been enqueued, before the host waits for it: host
+    // work placed there (copies on another stream) overlaps the chunk.
+    void run(const DropFn &on_drop = nullptr, const std::function<void()> &after_launch = nullptr)
     {
         start_pending();
         static const bool dbg = getenv("KQ_DEBUG") != nullptr;
@@ -414,6 +417,8 @@ public:
                 for (int c = 0; c < tune_.chunk; ++c)
                     KQ_HIP_CHECK(hipGraphLaunch(gx, st_));
             }
+            if (after_launch)
+                after_launch();
             const auto tb = clk::now();
             download_states();
             const auto tc = clk::now();
