@@ -113,6 +113,14 @@ static KuQCDStatus solve_ref_t(int slot, LF fat, LL lng, void **x, void *b, KuQC
     // overlap the solve.
     std::vector<char> copied(ns, 0);
     std::vector<std::pair<int, hipEvent_t>> ready;
+    struct EventGuard {  // events of drops not yet copied when an error ends the solve
+        std::vector<std::pair<int, hipEvent_t>> &v;
+        ~EventGuard()
+        {
+            for (const auto &kv : v)
+                (void)hipEventDestroy(kv.second);
+        }
+    } event_guard{ready};
     DevBuf stage2(sizeof(T) * 6 * (size_t)g.Vh);
     auto on_drop = [&](int, int k) {
         hipEvent_t e;
