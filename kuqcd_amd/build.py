@@ -41,6 +41,7 @@ SOURCES = [
     "kq_ops.cpp",
     "kq_gen.cpp",
     "kq_classes.cpp",
+    "kq_xfer.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

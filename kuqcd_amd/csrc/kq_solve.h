@@ -28,14 +28,17 @@ template <typename HR, typename VR> inline void upload_vec(const void *h, Vec<VR
     KQ_HIP_CHECK(hipStreamSynchronize(st));
 }
 
+// bytes from device memory d to h (pageable host, pinned host or device memory), synchronous with
+// respect to the host; pageable destinations go through pinned chunks (kq_xfer.cpp)
+void copy_to_host(void *h, const void *d, size_t bytes, hipStream_t st);
+
 template <typename VR, typename HR> inline void download_vec(Vec<VR> v, int r, void *h, DevBuf &stage, hipStream_t st)
 {
     const size_t n = (size_t)6 * v.Vh;
     if (stage.bytes() < sizeof(HR) * n)
         stage.alloc(sizeof(HR) * n);
     vec_soa_to_host<VR, HR>(v, r, stage.as<HR>(), st);
-    KQ_HIP_CHECK(hipMemcpyAsync(h, stage.as<void>(), sizeof(HR) * n, hipMemcpyDefault, st));
-    KQ_HIP_CHECK(hipStreamSynchronize(st));
+    copy_to_host(h, stage.as<void>(), sizeof(HR) * n, st);
 }
 
 // autotuned (nr, nt) for the hopping term
