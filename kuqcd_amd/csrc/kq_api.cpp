@@ -242,8 +242,7 @@ static void upload_links(const HR *h, GaugeSlot &slot, double scale, int prec, b
     DevBuf stage(sizeof(HR) * 72 * chunk);
     for (uint64_t i0 = 0; i0 < V; i0 += chunk) {
         const size_t n = (size_t)std::min<uint64_t>(chunk, V - i0);
-        KQ_HIP_CHECK(hipMemcpyAsync(stage.as<void>(), h + 72 * i0, sizeof(HR) * 72 * n, hipMemcpyDefault,
-                                    c.stream));
+        copy_from_host(stage.as<void>(), h + 72 * i0, sizeof(HR) * 72 * n, c.stream);
         if (prec == KUQCD_DOUBLE)
             gauge_host_to_soa<HR, double>(stage.as<HR>(), i0, n, slot.soa.as<double2>(), Vh, scale, c.stream);
         else

@@ -287,7 +287,7 @@ template <typename T, int NB, KuQCDReconstructType R> void GaugeField<T, NB, R>:
     kq::DevBuf stage(sizeof(T) * 72 * chunk);
     for (size_t i0 = 0; i0 < V; i0 += chunk) {
         const size_t n = std::min(chunk, V - i0);
-        KQ_HIP_CHECK(hipMemcpyAsync(stage.as<void>(), h + 72 * i0, sizeof(T) * 72 * n, hipMemcpyDefault, st));
+        kq::copy_from_host(stage.as<void>(), h + 72 * i0, sizeof(T) * 72 * n, st);
         kq::gauge_host_to_soa<T, T>(stage.as<T>(), i0, n, static_cast<kq::c2_t<T> *>(dev_), Vh, 1.0, st);
         KQ_HIP_CHECK(hipStreamSynchronize(st));
     }
