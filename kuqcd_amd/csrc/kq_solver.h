@@ -117,6 +117,17 @@ inline bool deferred_updates_on()
     return on;
 }
 
+// KQ_PAIR_REL=0 keeps every reliable update to the systems that reached their own threshold
+// (no partner joins an odd set; see CgmRunner::run)
+inline bool pair_reliable_on()
+{
+    static const bool on = [] {
+        const char *e = getenv("KQ_PAIR_REL");
+        return !(e != nullptr && e[0] == '0');
+    }();
+    return on;
+}
+
 // History slots for the deferred updates of up to nsys multishift systems: each slot records
 // kHistMax residuals of VR precision; the slots stay within 32 GiB (at 48^3x96 in FP64, four
 // slots), and systems beyond them update every iteration
@@ -464,6 +475,34 @@ public:
                 if (need_rel) {
                     c.pause = 0;
                     rel.push_back(sys);
+                }
+            }
+            // The FP64 operator reads its links once for a pair of systems (1.17x one system at
+            // 32^4), and an odd set leaves one system of a pair slot unused: the live system
+            // nearest its own threshold joins early if its residual has fallen at least halfway
+            // there (|r|^2 < delta |r_last|^2, against delta^2 for its own update)
+            if (reliable_ && rel.size() % 2 == 1 && pair_reliable_on()) {
+                int best = -1;
+                double bestq = delta_;
+                for (int sys = 0; sys < nsys_; ++sys) {
+                    const CgmState<S> &c = state_h_[sys];
+                    if (finalized_[sys] || c.stop || c.discard || c.r2max <= 0.0 ||
+                        std::find(rel.begin(), rel.end(), sys) != rel.end())
+                        continue;
+                    const double q = (double)c.rr / c.r2max;
+                    if (q < bestq) {
+                        bestq = q;
+                        best = sys;
+                    }
+                }
+                if (best >= 0) {
+                    CgmState<S> &c = state_h_[best];
+                    if (std::find(fold.begin(), fold.end(), best) == fold.end()) {
+                        c.maxTermFold = last_fold_[best];
+                        fold.push_back(best);
+                    }
+                    c.pause = 0;
+                    rel.push_back(best);
                 }
             }
             if (!fold.empty() || !rel.empty()) {
