@@ -376,11 +376,21 @@ struct Problem {
     double delta = 0.02;
 };
 
+// KQ_REFINE_L16=0 keeps the refinement of a 16-bit-link solve on its 16-bit fat links
+static bool refine_l16_on()
+{
+    static const bool on = [] {
+        const char *e = getenv("KQ_REFINE_L16");
+        return !(e != nullptr && e[0] == '0');
+    }();
+    return on;
+}
+
 // Batched single-shift refinement of the listed (rhs, shift) pairs from their
 // current FP64 solutions in x_out, to 0.5 * eps relative residual.
 template <class LF, class LL>
 static void refine_pairs(Problem &P, LF fat, LL lng, DevBuf &x_out, const std::vector<std::pair<int, int>> &need,
-                         RunnerTuning tune, KqSolveReport *rep)
+                         RunnerTuning tune, KqSolveReport *rep, int mode)
 {
     Context &c = ctx();
     const size_t n3 = (size_t)3 * P.g.Vh;
@@ -388,7 +398,7 @@ static void refine_pairs(Problem &P, LF fat, LL lng, DevBuf &x_out, const std::v
     if (P.o->nr > 0)
         tune.nr = P.o->nr;
     else
-        tune.nr_by_live = nr_table(P.slot, P.o->mode, nsys);
+        tune.nr_by_live = nr_table(P.slot, mode, nsys);
     CgmRunner<double, float, LF, LL> ref(P.g, P.parity, fat, lng, nsys, nsys, c.stream, tune);
     with_fp64_links(P.slot, [&](LinkD f, auto l) { ref.enable_reliable(f, l, P.delta); });
     for (int i = 0; i < nsys; ++i) {
@@ -553,7 +563,20 @@ static void run_mixed(Problem &P, LF fat, LL lng, DevBuf &x_out, KqSolveReport *
                         need.emplace_back(r, k);
             if (need.empty())
                 break;
-            refine_pairs<LF, LL>(P, fat, lng, x_out, need, tune, rep);
+            if (P.o->mode == KQ_MODE_MIXED_FP16 && refine_l16_on()) {
+                // 16-bit fat links misrepresent the lowest modes of the lightest systems (why
+                // those stall): what the 16-bit solve left above the target is refined with FP32
+                // fat links and the same 16-bit long links
+                with_mixed_links(P.slot, KQ_MODE_MIXED_FP32_L16, [&](auto f, auto l) {
+                    RunnerTuning t2;
+                    t2.nt = P.o->nt > 0 ? P.o->nt : tuned_launch(P.slot, KQ_MODE_MIXED_FP32_L16, nrhs).second;
+                    t2.chunk = tune.chunk;
+                    set_caps(t2, P.slot, KQ_MODE_MIXED_FP32_L16, true);
+                    refine_pairs<decltype(f), decltype(l)>(P, f, l, x_out, need, t2, rep, KQ_MODE_MIXED_FP32_L16);
+                });
+            } else {
+                refine_pairs<LF, LL>(P, fat, lng, x_out, need, tune, rep, P.o->mode);
+            }
             for (int r = 0; r < nrhs; ++r)
                 rep[r].refine_rounds = round + 1;
             verify(need);
