@@ -23,11 +23,14 @@ typedef float kq_v2f __attribute__((ext_vector_type(2)));
 typedef float kq_v4f __attribute__((ext_vector_type(4)));
 typedef unsigned int kq_v4u __attribute__((ext_vector_type(4)));
 typedef unsigned int kq_v2u __attribute__((ext_vector_type(2)));
+// 12-byte word (one dwordx3 load), 4-byte aligned: records of 12 bytes sit back to back
+typedef unsigned int kq_v3u __attribute__((ext_vector_type(3), aligned(4)));
 __device__ inline kq_v2d ld_nt(const kq_v2d *p) { return __builtin_nontemporal_load(p); }
 __device__ inline kq_v4f ld_nt(const kq_v4f *p) { return __builtin_nontemporal_load(p); }
 __device__ inline kq_v2f ld_nt(const kq_v2f *p) { return __builtin_nontemporal_load(p); }
 __device__ inline kq_v4u ld_nt(const kq_v4u *p) { return __builtin_nontemporal_load(p); }
 __device__ inline kq_v2u ld_nt(const kq_v2u *p) { return __builtin_nontemporal_load(p); }
+__device__ inline kq_v3u ld_nt(const kq_v3u *p) { return __builtin_nontemporal_load(p); }
 __device__ inline float ld_nt(const float *p) { return __builtin_nontemporal_load(p); }
 
 // Streaming store of one complex number: the hopping term's output is not read
@@ -221,7 +224,8 @@ struct LinkH {
 
 // 16-bit links, 12 reals (rows 0 and 1 of an SU(3) matrix V as fixed point
 // q / 32767), one signed FP32 scale b per link: U = b * V with row 2 of V =
-// conj(row0 x row1). Chunks: 16 B (complex 0..3), 8 B (complex 4, 5), 4 B (scale).
+// conj(row0 x row1). Chunks: 16 B (complex 0..3), then one 12 B record of complex 4, 5 and the
+// scale (its bits), so a link takes two loads.
 struct LinkH12 {
     using R = float;
     static constexpr int kBytes = 28;
@@ -230,8 +234,7 @@ struct LinkH12 {
     uint32_t nb64;
     struct Raw {
         kq_v4u c0;
-        kq_v2u c1;
-        float a;
+        kq_v3u c1;
     };
     __device__ inline Raw fetch(int q, int mu, uint32_t s) const
     {
@@ -239,16 +242,15 @@ struct LinkH12 {
         const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
         const uint32_t l = s & 63;
         r.c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
-        r.c1 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 1024) + l);
-        r.a = ld_nt(reinterpret_cast<const float *>(b + 1536) + l);
+        r.c1 = ld_nt(reinterpret_cast<const kq_v3u *>(b + 1024 + 12 * l));
         return r;
     }
     __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const { unpack(fetch(q, mu, s), u); }
     __device__ inline void unpack(const Raw &r, float2 u[9]) const
     {
         const kq_v4u c0 = r.c0;
-        const kq_v2u c1 = r.c1;
-        const float a = r.a;
+        const kq_v3u c1 = r.c1;
+        const float a = __uint_as_float(c1.z);
         constexpr float kQ = 1.0f / 32767.0f;
         float2 v[6];
         v[0] = q2f(c0.x);
@@ -385,7 +387,7 @@ struct LinkF12C {
 };
 
 // The same in 16 bits: rows 0 and 1 of V as fixed point q / 32767 and w in FP32.
-// Chunks: 16 B (complex 0..3), 8 B (complex 4, 5), 8 B (w): 32 B.
+// Chunks: 16 B (complex 0..3), 16 B (complex 4, 5 and the bits of w): 32 B in two loads.
 struct LinkH12C {
     using R = float;
     static constexpr int kBytes = 32;
@@ -394,8 +396,7 @@ struct LinkH12C {
     uint32_t nb64;
     struct Raw {
         kq_v4u c0;
-        kq_v2u c1;
-        kq_v2f w;
+        kq_v4u c1;
     };
     __device__ inline Raw fetch(int q, int mu, uint32_t s) const
     {
@@ -403,16 +404,15 @@ struct LinkH12C {
         const char *b = p + link_block_offset(q, mu, s, nb64, kBlock);
         const uint32_t l = s & 63;
         r.c0 = ld_nt(reinterpret_cast<const kq_v4u *>(b) + l);
-        r.c1 = ld_nt(reinterpret_cast<const kq_v2u *>(b + 1024) + l);
-        r.w = ld_nt(reinterpret_cast<const kq_v2f *>(b + 1536) + l);
+        r.c1 = ld_nt(reinterpret_cast<const kq_v4u *>(b + 1024) + l);
         return r;
     }
     __device__ inline void load(int q, int mu, uint32_t s, float2 u[9]) const { unpack(fetch(q, mu, s), u); }
     __device__ inline void unpack(const Raw &r, float2 u[9]) const
     {
         const kq_v4u c0 = r.c0;
-        const kq_v2u c1 = r.c1;
-        const kq_v2f w = r.w;
+        const kq_v4u c1 = r.c1;
+        const kq_v2f w = {__uint_as_float(c1.z), __uint_as_float(c1.w)};
         constexpr float kQ = 1.0f / 32767.0f;
         const unsigned int h[6] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y};
         float2 v[6];

@@ -193,8 +193,11 @@ __global__ void k_canon_to_linkH12(const double2 *canon, char *dst, uint32_t Vh,
     char *b = dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkH12::kBlock);
     const uint32_t l = s & 63;
     reinterpret_cast<uint4 *>(b)[l] = make_uint4(h[0], h[1], h[2], h[3]);
-    reinterpret_cast<uint2 *>(b + 1024)[l] = make_uint2(h[4], h[5]);
-    reinterpret_cast<float *>(b + 1536)[l] = (float)bsc;
+    // complex 4, 5 and the scale as one 12-byte record
+    unsigned int *rec = reinterpret_cast<unsigned int *>(b + 1024 + 12 * l);
+    rec[0] = h[4];
+    rec[1] = h[5];
+    rec[2] = __float_as_uint((float)bsc);
 }
 
 // U = w V with w = principal cube root of det U, V = U / w; true when the rebuilt w (V0, V1,
@@ -265,8 +268,8 @@ __global__ void k_canon_to_linkH12C(const double2 *canon, char *dst, uint32_t Vh
     char *b = dst + link_block_offset(qm / 4, qm % 4, s, nb64, LinkH12C::kBlock);
     const uint32_t l = s & 63;
     reinterpret_cast<uint4 *>(b)[l] = make_uint4(h[0], h[1], h[2], h[3]);
-    reinterpret_cast<uint2 *>(b + 1024)[l] = make_uint2(h[4], h[5]);
-    reinterpret_cast<float2 *>(b + 1536)[l] = make_float2((float)w.x, (float)w.y);
+    reinterpret_cast<uint4 *>(b + 1024)[l] =
+        make_uint4(h[4], h[5], __float_as_uint((float)w.x), __float_as_uint((float)w.y));
 }
 
 template <typename C2> static void canon_to_linkF12C_t(const C2 *canon, char *dst, uint32_t Vh, int *fail,
